@@ -1,0 +1,104 @@
+/* Extended C interface of the MI355X LP partitioner (libkahip.so).
+ *
+ * Status-returning entry points: 0 on success, non-zero on failure with a message available from
+ * kahip_last_error() (thread-local). All arrays are host arrays unless a kahip_graph_t is used,
+ * whose CSR lives in GPU memory. Conventions as in kaHIP_interface.h.
+ */
+#ifndef KAHIP_AMD_H
+#define KAHIP_AMD_H
+
+#include "kaHIP_interface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kahip_graph* kahip_graph_t;
+
+/* Parameters of one partitioning call; kahip_default_params() fills the FASTSOCIAL values. */
+typedef struct {
+    int k;
+    double imbalance;
+    int seed;
+    int cluster_rounds;  /* LP clustering rounds per level (reference label_iterations = 10) */
+    int refine_rounds;   /* LP refinement rounds per level (label_iterations_refinement = 25) */
+    int subrounds;       /* sub-rounds per synchronous round, 1..8 */
+    int cluster_factor;  /* cluster bound B = ceil(U / cluster_factor) (reference 18) */
+    int stop_factor;     /* coarsening continues while n_coarse >= stop_factor * k (5000) */
+    int ip_repetitions;  /* initial-partitioning repetitions; -1 = the reference's count */
+    int ip_threads;      /* CPU threads of the initial partitioner; 0 = automatic */
+    int bins[6];         /* upper degree bounds of the LP kernel bins; bins[4] <= 512 */
+    int profile;         /* record per-launch timings (kahip_profile_json) */
+} kahip_params;
+
+#define KAHIP_MAX_LEVELS 64
+
+typedef struct {
+    long long cut;
+    double balance;
+    unsigned upper_bound;   /* U */
+    unsigned cluster_bound; /* B */
+    int levels;             /* number of contractions */
+    int level_n[KAHIP_MAX_LEVELS + 1];
+    long long level_m2[KAHIP_MAX_LEVELS + 1];
+    double ms_coarsen, ms_initial, ms_refine, ms_cut;
+    long long edges_cluster, edges_contract, edges_refine, edges_cut;
+    long long max_block_weight;
+} kahip_stats;
+
+void kahip_default_params(kahip_params* p);
+const char* kahip_last_error(void);
+
+/* Status-returning kaffpa (no reference counterpart: the reference's kaffpa returns void,
+ * interface/kaHIP_interface.cpp:146-166). stats may be NULL. */
+int kaffpa_gpu_ex(int n, const int* vwgt, const int* xadj, const int* adjcwgt, const int* adjncy,
+                  const kahip_params* params, int* edgecut, int* part, kahip_stats* stats);
+
+/* Device-resident graphs. */
+int kahip_graph_from_host(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                          const int* adjwgt, kahip_graph_t* out);
+/* R-MAT generated on the GPU: 2^scale nodes, `samples` samples with quadrant probabilities
+ * (a, b, c, 1-a-b-c); self-loops dropped, symmetrised, duplicates merged with weight =
+ * multiplicity, rows sorted. The stream of samples is the counter-based one documented in
+ * kahip_amd/graphs.py. */
+int kahip_graph_rmat(int scale, long long samples, unsigned long long seed, double a, double b,
+                     double c, kahip_graph_t* out);
+int kahip_graph_info(kahip_graph_t g, int* n, long long* m2, long long* total_vwgt,
+                     int* max_degree);
+int kahip_graph_to_host(kahip_graph_t g, int* xadj, int* adjncy, int* vwgt, int* adjwgt);
+void kahip_graph_free(kahip_graph_t g);
+
+/* kaffpa's pipeline on a device graph. part: host array of n ints (may be NULL). */
+int kahip_partition(kahip_graph_t g, const kahip_params* params, int* part, kahip_stats* stats);
+/* JSON object {name: {launches, ms, bytes, edges}} of the last profiled call on this thread. */
+int kahip_profile_json(char* buf, int len);
+
+/* Fine-grained steps (host arrays in and out; one GPU round trip each). */
+/* Synchronous LP clustering + first-appearance relabel (kernel (a) of the path). */
+int kahip_lp_cluster(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                     const int* adjwgt, int bound, int rounds, int subrounds, unsigned seed,
+                     int* cluster, int* num_clusters, int* moves_per_round);
+/* First-appearance relabel of arbitrary labels in [0, n). */
+int kahip_remap(int n, const int* label, int* map, int* num_clusters);
+/* Quotient graph of a clustering with ids 0..nc-1; cadjncy / cadjwgt hold xadj[n] entries. */
+int kahip_contract(int n, const int* xadj, const int* adjncy, const int* vwgt, const int* adjwgt,
+                   const int* map, int nc, int* cxadj, int* cadjncy, int* cadjwgt, int* cvwgt);
+/* Synchronous LP refinement of part (in/out) under block bound `bound`. */
+int kahip_lp_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, int bound, int rounds, int subrounds, unsigned seed,
+                    int* part, int* moves_per_round);
+int kahip_edge_cut(int n, const int* xadj, const int* adjncy, const int* adjwgt, const int* part,
+                   long long* cut);
+int kahip_block_weights(int n, const int* vwgt, int k, const int* part, long long* weights);
+int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_part,
+                  int* fine_part);
+/* CPU initial partitioner of the coarsest graph (no GPU). */
+int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                            const int* adjwgt, int k, long long block_bound, int repetitions,
+                            unsigned long long seed, int* part);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

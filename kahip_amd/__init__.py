@@ -1,0 +1,354 @@
+"""kahip_amd: KaHIP's label-propagation coarsening / refinement path on MI355X (gfx950).
+
+The compute lives in ``kahip_amd/lib/libkahip.so`` (HIP kernels + C++ host driver behind a C ABI,
+see include/kaHIP_interface.h and include/kahip_amd.h). This module is a thin ctypes layer:
+
+* ``kaffpa(vwgt, xadj, adjcwgt, adjncy, nparts, imbalance, suppress_output, seed, mode)``
+  mirrors the reference's Python binding (misc/pymodule/kahip.cpp:4-43,
+  python/kahip/__init__.py:14) and returns ``(edgecut, blocks)``.
+* ``partition(...)``, ``Graph`` (a device-resident CSR) and the fine-grained steps
+  (``lp_cluster``, ``contract``, ``lp_refine``, ``edge_cut``, ...) expose the pipeline's parts.
+
+There is no CPU fallback: if the library is missing or a GPU call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from .graphs import (gen_rgg, gen_rmat, read_metis, read_partition, write_metis,  # noqa: F401
+                     write_partition, check_graph)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libkahip.so")
+
+FAST, ECO, STRONG, FASTSOCIAL, ECOSOCIAL, STRONGSOCIAL = range(6)
+MAX_LEVELS = 64
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class KahipError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("k", C.c_int), ("imbalance", C.c_double), ("seed", C.c_int),
+                ("cluster_rounds", C.c_int), ("refine_rounds", C.c_int), ("subrounds", C.c_int),
+                ("cluster_factor", C.c_int), ("stop_factor", C.c_int),
+                ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
+                ("profile", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("cut", C.c_longlong), ("balance", C.c_double), ("upper_bound", C.c_uint),
+                ("cluster_bound", C.c_uint), ("levels", C.c_int),
+                ("level_n", C.c_int * (MAX_LEVELS + 1)),
+                ("level_m2", C.c_longlong * (MAX_LEVELS + 1)),
+                ("ms_coarsen", C.c_double), ("ms_initial", C.c_double),
+                ("ms_refine", C.c_double), ("ms_cut", C.c_double),
+                ("edges_cluster", C.c_longlong), ("edges_contract", C.c_longlong),
+                ("edges_refine", C.c_longlong), ("edges_cut", C.c_longlong),
+                ("max_block_weight", C.c_longlong)]
+
+    def as_dict(self):
+        L = self.levels
+        return {
+            "cut": int(self.cut), "balance": float(self.balance),
+            "upper_bound": int(self.upper_bound), "cluster_bound": int(self.cluster_bound),
+            "levels": L, "level_n": [int(x) for x in self.level_n[:L + 1]],
+            "level_m2": [int(x) for x in self.level_m2[:L + 1]],
+            "ms_coarsen": self.ms_coarsen, "ms_initial": self.ms_initial,
+            "ms_refine": self.ms_refine, "ms_cut": self.ms_cut,
+            "edges_cluster": int(self.edges_cluster), "edges_contract": int(self.edges_contract),
+            "edges_refine": int(self.edges_refine), "edges_cut": int(self.edges_cut),
+            "max_block_weight": int(self.max_block_weight),
+        }
+
+
+# Every symbol the headers in include/ declare (checked by the tests).
+EXPORTS = [
+    "kaffpa", "kaffpa_balance", "kaffpa_balance_NE", "kahip_default_params", "kahip_last_error",
+    "kaffpa_gpu_ex", "kahip_graph_from_host", "kahip_graph_rmat", "kahip_graph_info",
+    "kahip_graph_to_host", "kahip_graph_free", "kahip_partition", "kahip_profile_json",
+    "kahip_lp_cluster", "kahip_remap", "kahip_contract", "kahip_lp_refine", "kahip_edge_cut",
+    "kahip_block_weights", "kahip_project", "kahip_initial_partition",
+]
+
+_lib = None
+
+
+def build(verbose=False):
+    from . import build as _b
+    return _b.build(verbose=verbose)
+
+
+def lib():
+    """Loads libkahip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KahipError(f"{LIB_PATH} is missing: run `python -m kahip_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    ip = C.POINTER(C.c_int)
+    G = C.c_void_p
+    L.kahip_default_params.argtypes = [C.POINTER(Params)]
+    L.kahip_default_params.restype = None
+    L.kahip_last_error.restype = C.c_char_p
+    L.kaffpa_gpu_ex.argtypes = [C.c_int, ip, _i32p, ip, _i32p, C.POINTER(Params), ip, _i32p,
+                                C.POINTER(Stats)]
+    L.kaffpa.argtypes = [ip, ip, ip, ip, ip, ip, C.POINTER(C.c_double), C.c_bool, C.c_int,
+                         C.c_int, ip, ip]
+    L.kaffpa.restype = None
+    L.kaffpa_balance.argtypes = [ip, ip, ip, ip, ip, ip, C.POINTER(C.c_double), C.c_bool,
+                                 C.c_bool, C.c_int, C.c_int, ip, ip]
+    L.kaffpa_balance.restype = None
+    L.kaffpa_balance_NE.argtypes = L.kaffpa.argtypes
+    L.kaffpa_balance_NE.restype = None
+    L.kahip_graph_from_host.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.POINTER(G)]
+    L.kahip_graph_rmat.argtypes = [C.c_int, C.c_longlong, C.c_ulonglong, C.c_double,
+                                   C.c_double, C.c_double, C.POINTER(G)]
+    L.kahip_graph_info.argtypes = [G, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
+    L.kahip_graph_to_host.argtypes = [G, ip, ip, ip, ip]
+    L.kahip_graph_free.argtypes = [G]
+    L.kahip_graph_free.restype = None
+    L.kahip_partition.argtypes = [G, C.POINTER(Params), ip, C.POINTER(Stats)]
+    L.kahip_profile_json.argtypes = [C.c_char_p, C.c_int]
+    L.kahip_lp_cluster.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_int, C.c_int,
+                                   C.c_uint, _i32p, ip, ip]
+    L.kahip_remap.argtypes = [C.c_int, _i32p, _i32p, ip]
+    L.kahip_contract.argtypes = [C.c_int, _i32p, _i32p, ip, ip, _i32p, C.c_int, _i32p, _i32p,
+                                 _i32p, _i32p]
+    L.kahip_lp_refine.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_uint, _i32p, ip]
+    L.kahip_edge_cut.argtypes = [C.c_int, _i32p, _i32p, ip, _i32p, C.POINTER(C.c_longlong)]
+    L.kahip_block_weights.argtypes = [C.c_int, ip, C.c_int, _i32p, _i64p]
+    L.kahip_project.argtypes = [C.c_int, _i32p, C.c_int, _i32p, _i32p]
+    L.kahip_initial_partition.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
+                                          C.c_int, C.c_ulonglong, _i32p]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise KahipError(lib().kahip_last_error().decode())
+
+
+def _p(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().kahip_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "bins":
+            for i, b in enumerate(v):
+                p.bins[i] = b
+        else:
+            setattr(p, k, v)
+    return p
+
+
+# ----------------------------------------------------------------------------- drop-in API
+def kaffpa(vwgt, xadj, adjcwgt, adjncy, nparts, imbalance, suppress_output, seed, mode):
+    """Same arguments and return value as the reference's ``kahip.kaffpa``: ``(edgecut, blocks)``.
+
+    Raises KahipError on failure (the reference has no error channel)."""
+    xadj = _i32(xadj)
+    adjncy = _i32(adjncy)
+    n = len(xadj) - 1
+    vw = None if vwgt is None or len(vwgt) == 0 else _i32(vwgt)
+    ew = None if adjcwgt is None or len(adjcwgt) == 0 else _i32(adjcwgt)
+    part = np.zeros(max(n, 1), np.int32)
+    cut = C.c_int(0)
+    p = default_params(k=int(nparts), imbalance=float(imbalance), seed=int(seed))
+    _keep = (vw, ew)
+    _check(lib().kaffpa_gpu_ex(n, _p(vw), xadj, _p(ew), adjncy, C.byref(p), C.byref(cut), part,
+                               None))
+    del _keep
+    return int(cut.value), part[:n].tolist()
+
+
+def partition(xadj, adjncy, vwgt=None, adjwgt=None, *, k, imbalance=0.03, seed=0, **params):
+    """kaffpa's pipeline with explicit parameters; returns (part ndarray, stats dict)."""
+    xadj = _i32(xadj)
+    adjncy = _i32(adjncy)
+    n = len(xadj) - 1
+    vw = None if vwgt is None else _i32(vwgt)
+    ew = None if adjwgt is None else _i32(adjwgt)
+    part = np.zeros(max(n, 1), np.int32)
+    cut = C.c_int(0)
+    st = Stats()
+    p = default_params(k=k, imbalance=imbalance, seed=seed, **params)
+    _check(lib().kaffpa_gpu_ex(n, _p(vw), xadj, _p(ew), adjncy, C.byref(p), C.byref(cut), part,
+                               C.byref(st)))
+    return part[:n], st.as_dict()
+
+
+class Graph:
+    """A CSR graph resident in GPU memory."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def from_host(cls, xadj, adjncy, vwgt=None, adjwgt=None):
+        h = C.c_void_p()
+        xadj, adjncy = _i32(xadj), _i32(adjncy)
+        vw = None if vwgt is None else _i32(vwgt)
+        ew = None if adjwgt is None else _i32(adjwgt)
+        _check(lib().kahip_graph_from_host(len(xadj) - 1, xadj, adjncy, _p(vw), _p(ew),
+                                           C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, samples, seed, a=0.57, b=0.19, c=0.19):
+        h = C.c_void_p()
+        _check(lib().kahip_graph_rmat(scale, samples, seed, a, b, c, C.byref(h)))
+        return cls(h)
+
+    def info(self):
+        n, md = C.c_int(), C.c_int()
+        m2, tw = C.c_longlong(), C.c_longlong()
+        _check(lib().kahip_graph_info(self._h, C.byref(n), C.byref(m2), C.byref(tw),
+                                      C.byref(md)))
+        return {"n": n.value, "m2": m2.value, "total_vwgt": tw.value, "max_degree": md.value}
+
+    def to_host(self):
+        inf = self.info()
+        xadj = np.zeros(inf["n"] + 1, np.int32)
+        adj = np.zeros(max(inf["m2"], 1), np.int32)
+        ew = np.zeros(max(inf["m2"], 1), np.int32)
+        vw = np.zeros(max(inf["n"], 1), np.int32)
+        ip = C.POINTER(C.c_int)
+        _check(lib().kahip_graph_to_host(self._h, xadj.ctypes.data_as(ip), adj.ctypes.data_as(ip),
+                                         vw.ctypes.data_as(ip), ew.ctypes.data_as(ip)))
+        return xadj, adj[:inf["m2"]], vw[:inf["n"]], ew[:inf["m2"]]
+
+    def partition(self, k, imbalance=0.03, seed=0, want_part=True, **params):
+        n = self.info()["n"]
+        part = np.zeros(max(n, 1), np.int32) if want_part else None
+        st = Stats()
+        p = default_params(k=k, imbalance=imbalance, seed=seed, **params)
+        _check(lib().kahip_partition(self._h, C.byref(p),
+                                     part.ctypes.data_as(C.POINTER(C.c_int)) if want_part else None,
+                                     C.byref(st)))
+        return (part[:n] if want_part else None), st.as_dict()
+
+    def free(self):
+        if self._h:
+            lib().kahip_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def profile():
+    """Per-launch-name timing summary of the last profiled call on this thread."""
+    need = lib().kahip_profile_json(None, 0)
+    buf = C.create_string_buffer(need + 16)
+    lib().kahip_profile_json(buf, len(buf))
+    return json.loads(buf.value.decode() or "{}")
+
+
+# ----------------------------------------------------------------------------- single steps
+def lp_cluster(xadj, adjncy, vwgt, adjwgt, bound, rounds, subrounds, seed):
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    out = np.zeros(max(n, 1), np.int32)
+    nc = C.c_int(0)
+    moves = np.zeros(max(rounds, 1), np.int32)
+    _check(lib().kahip_lp_cluster(n, xadj, adjncy, _p(vwgt), _p(adjwgt), int(bound), rounds,
+                                  subrounds, seed & 0xFFFFFFFF, out, C.byref(nc),
+                                  moves.ctypes.data_as(C.POINTER(C.c_int))))
+    return out[:n], nc.value, moves[:rounds]
+
+
+def remap(label):
+    label = _i32(label)
+    n = len(label)
+    out = np.zeros(max(n, 1), np.int32)
+    nc = C.c_int(0)
+    _check(lib().kahip_remap(n, label if n else np.zeros(1, np.int32), out, C.byref(nc)))
+    return out[:n], nc.value
+
+
+def contract(xadj, adjncy, vwgt, adjwgt, cluster, nc):
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    m2 = int(xadj[-1])
+    cx = np.zeros(nc + 1, np.int32)
+    ca = np.zeros(max(m2, 1), np.int32)
+    cw = np.zeros(max(m2, 1), np.int32)
+    cv = np.zeros(max(nc, 1), np.int32)
+    cl = _i32(cluster) if n else np.zeros(1, np.int32)
+    _check(lib().kahip_contract(n, xadj, adjncy, _p(vwgt), _p(adjwgt), cl, nc, cx, ca, cw, cv))
+    mc = int(cx[-1])
+    return cx, ca[:mc], cw[:mc], cv[:nc]
+
+
+def lp_refine(xadj, adjncy, vwgt, adjwgt, k, bound, rounds, subrounds, seed, part):
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    p = np.array(part, dtype=np.int32) if n else np.zeros(1, np.int32)
+    moves = np.zeros(max(rounds, 1), np.int32)
+    _check(lib().kahip_lp_refine(n, xadj, adjncy, _p(vwgt), _p(adjwgt), k, int(bound), rounds,
+                                 subrounds, seed & 0xFFFFFFFF, p,
+                                 moves.ctypes.data_as(C.POINTER(C.c_int))))
+    return p[:n], moves[:rounds]
+
+
+def edge_cut(xadj, adjncy, adjwgt, part):
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    cut = C.c_longlong(0)
+    pa = _i32(part) if n else np.zeros(1, np.int32)
+    _check(lib().kahip_edge_cut(n, xadj, adjncy if len(adjncy) else np.zeros(1, np.int32),
+                                _p(adjwgt), pa, C.byref(cut)))
+    return int(cut.value)
+
+
+def block_weights(vwgt, k, part):
+    part = _i32(part)
+    n = len(part)
+    w = np.zeros(k, np.int64)
+    _check(lib().kahip_block_weights(n, _p(vwgt), k, part if n else np.zeros(1, np.int32), w))
+    return w
+
+
+def project(mapping, coarse_part):
+    mapping, coarse_part = _i32(mapping), _i32(coarse_part)
+    out = np.zeros(max(len(mapping), 1), np.int32)
+    _check(lib().kahip_project(len(mapping), mapping if len(mapping) else np.zeros(1, np.int32),
+                               len(coarse_part),
+                               coarse_part if len(coarse_part) else np.zeros(1, np.int32), out))
+    return out[:len(mapping)]
+
+
+def initial_partition(xadj, adjncy, vwgt, adjwgt, k, block_bound, repetitions=-1, seed=0):
+    """The CPU initial partitioner of the coarsest graph (runs without a GPU)."""
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    out = np.zeros(max(n, 1), np.int32)
+    _check(lib().kahip_initial_partition(n, xadj, adjncy if len(adjncy) else np.zeros(1, np.int32),
+                                         _p(vwgt), _p(adjwgt), k, int(block_bound), repetitions,
+                                         seed, out))
+    return out[:n]

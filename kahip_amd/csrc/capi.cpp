@@ -1,0 +1,555 @@
+// extern "C" entry points of libkahip.so (declared in include/kaHIP_interface.h and
+// include/kahip_amd.h). Host arrays are copied to the GPU, the work runs on a per-call stream,
+// results are copied back; no state survives a call except the thread-local last error and
+// last profile summary.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "driver.hpp"
+#include "gen.hpp"
+#include "kahip_amd.h"
+
+using namespace kahip;
+
+struct kahip_graph {
+    OwnedGraph og;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local std::string g_last_profile = "{}";
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return 1;
+}
+
+PartitionParams to_params(const kahip_params* q) {
+    kahip_params d;
+    kahip_default_params(&d);
+    if (!q) q = &d;
+    PartitionParams p;
+    p.k = q->k;
+    p.imbalance = q->imbalance;
+    p.seed = q->seed;
+    p.cluster_rounds = q->cluster_rounds;
+    p.refine_rounds = q->refine_rounds;
+    p.subrounds = q->subrounds;
+    p.cluster_factor = q->cluster_factor;
+    p.stop_factor = q->stop_factor;
+    p.ip_repetitions = q->ip_repetitions;
+    p.ip_threads = q->ip_threads;
+    for (int b = 0; b < kNumBins; ++b) p.bins.hi[b] = q->bins[b];
+    p.profile = q->profile != 0;
+    KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
+    for (int b = 0; b + 1 < kNumBins; ++b)
+        KH_REQUIRE(p.bins.hi[b] < p.bins.hi[b + 1], "bins must be strictly increasing");
+    KH_REQUIRE(p.bins.hi[0] == 8 && p.bins.hi[1] == 16 && p.bins.hi[2] == 32 &&
+                   p.bins.hi[3] == 64,
+               "bins[0..3] are the lane-group widths 8, 16, 32, 64");
+    KH_REQUIRE(p.cluster_rounds >= 0 && p.cluster_rounds <= kMaxRounds, "bad cluster_rounds");
+    KH_REQUIRE(p.refine_rounds >= 0 && p.refine_rounds <= kMaxRounds, "bad refine_rounds");
+    KH_REQUIRE(p.cluster_factor >= 1, "cluster_factor must be >= 1");
+    return p;
+}
+
+void check_csr(int n, const int* xadj, const int* adjncy) {
+    KH_REQUIRE(n >= 0, "n must be non-negative");
+    KH_REQUIRE(xadj != nullptr, "xadj is NULL");
+    KH_REQUIRE(xadj[0] == 0, "xadj[0] must be 0");
+    for (int v = 0; v < n; ++v) KH_REQUIRE(xadj[v + 1] >= xadj[v], "xadj must be non-decreasing");
+    const long long m2 = xadj[n];
+    KH_REQUIRE(m2 == 0 || adjncy != nullptr, "adjncy is NULL");
+    for (long long e = 0; e < m2; ++e)
+        KH_REQUIRE(adjncy[e] >= 0 && adjncy[e] < n, "adjncy entry out of range");
+}
+
+std::string profile_json(const Profiler& prof) {
+    std::ostringstream os;
+    os << "{";
+    bool first = true;
+    for (const auto& kv : prof.summarize()) {
+        if (!first) os << ",";
+        first = false;
+        os << "\"" << kv.first << "\":{\"launches\":" << kv.second.launches
+           << ",\"ms\":" << kv.second.ms << ",\"bytes\":" << kv.second.bytes
+           << ",\"edges\":" << kv.second.edges << "}";
+    }
+    os << "}";
+    return os.str();
+}
+
+void fill_stats(const PartitionStats& s, kahip_stats* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->cut = s.cut;
+    o->balance = s.balance;
+    o->upper_bound = s.upper_bound;
+    o->cluster_bound = s.cluster_bound;
+    o->levels = s.levels;
+    for (int i = 0; i <= kMaxLevels; ++i) {
+        o->level_n[i] = s.level_n[i];
+        o->level_m2[i] = s.level_m2[i];
+    }
+    o->ms_coarsen = s.ms_coarsen;
+    o->ms_initial = s.ms_initial;
+    o->ms_refine = s.ms_refine;
+    o->ms_cut = s.ms_cut;
+    o->edges_cluster = s.edges_cluster;
+    o->edges_contract = s.edges_contract;
+    o->edges_refine = s.edges_refine;
+    o->edges_cut = s.edges_cut;
+    o->max_block_weight = s.max_block_weight;
+}
+
+// Greedy CPU rebalancing for kaffpa_balance(perfectly_balance): while a block exceeds `bound`,
+// move the node of an overloaded block whose move to a block with room loses the least cut
+// (neighbouring blocks first, otherwise the lightest block). Stands in for the reference's
+// cycle_refinement post-pass (interface/kaHIP_interface.cpp:123-132), which is out of scope.
+long long rebalance(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, long long bound, int* part) {
+    std::vector<long long> W(k, 0);
+    auto vw = [&](int v) -> long long { return vwgt ? vwgt[v] : 1; };
+    auto ew = [&](long long e) -> long long { return adjwgt ? adjwgt[e] : 1; };
+    for (int v = 0; v < n; ++v) W[part[v]] += vw(v);
+    std::vector<long long> conn(k, 0);
+    std::vector<int> touched;
+    for (int guard = 0; guard < 4 * n + 16; ++guard) {
+        int over = -1;
+        for (int b = 0; b < k; ++b)
+            if (W[b] > bound && (over < 0 || W[b] > W[over])) over = b;
+        if (over < 0) break;
+        int lightest = (int)(std::min_element(W.begin(), W.end()) - W.begin());
+        long long best_loss = 0;
+        int best_v = -1, best_t = -1;
+        for (int v = 0; v < n; ++v) {
+            if (part[v] != over) continue;
+            for (long long e = xadj[v]; e < xadj[v + 1]; ++e) {
+                const int b = part[adjncy[e]];
+                if (conn[b] == 0) touched.push_back(b);
+                conn[b] += ew(e);
+            }
+            const long long own = conn[over];
+            int t = -1;
+            long long tc = -1;
+            for (int b : touched)
+                if (b != over && W[b] + vw(v) <= bound && conn[b] > tc) {
+                    t = b;
+                    tc = conn[b];
+                }
+            if (t < 0 && W[lightest] + vw(v) <= bound && lightest != over) {
+                t = lightest;
+                tc = conn[lightest];
+            }
+            for (int b : touched) conn[b] = 0;
+            touched.clear();
+            if (t < 0) continue;
+            const long long loss = own - tc;
+            if (best_v < 0 || loss < best_loss) {
+                best_loss = loss;
+                best_v = v;
+                best_t = t;
+            }
+        }
+        if (best_v < 0) break;
+        W[over] -= vw(best_v);
+        W[best_t] += vw(best_v);
+        part[best_v] = best_t;
+    }
+    long long cut = 0;
+    for (int v = 0; v < n; ++v)
+        for (long long e = xadj[v]; e < xadj[v + 1]; ++e)
+            if (part[v] != part[adjncy[e]]) cut += ew(e);
+    return cut / 2;
+}
+
+int run_kaffpa(int n, const int* vwgt, const int* xadj, const int* adjcwgt, const int* adjncy,
+               const kahip_params* params, int* edgecut, int* part, kahip_stats* stats) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        PartitionParams p = to_params(params);
+        Engine eng;
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, vwgt, adjcwgt, g, eng.stream());
+        PartitionStats st;
+        eng.partition(g.g, p, part, &st);
+        if (p.profile) g_last_profile = profile_json(eng.prof);
+        if (edgecut) *edgecut = (int)st.cut;
+        fill_stats(st, stats);
+    });
+}
+
+void report(const char* fn, int* edgecut) {
+    std::fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
+    if (edgecut) *edgecut = -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kahip_default_params(kahip_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    PartitionParams d;
+    p->k = d.k;
+    p->imbalance = d.imbalance;
+    p->seed = d.seed;
+    p->cluster_rounds = d.cluster_rounds;
+    p->refine_rounds = d.refine_rounds;
+    p->subrounds = d.subrounds;
+    p->cluster_factor = d.cluster_factor;
+    p->stop_factor = d.stop_factor;
+    p->ip_repetitions = d.ip_repetitions;
+    p->ip_threads = d.ip_threads;
+    for (int b = 0; b < kNumBins; ++b) p->bins[b] = d.bins.hi[b];
+    p->profile = 0;
+}
+
+const char* kahip_last_error(void) { return g_last_error.c_str(); }
+
+int kaffpa_gpu_ex(int n, const int* vwgt, const int* xadj, const int* adjcwgt, const int* adjncy,
+                  const kahip_params* params, int* edgecut, int* part, kahip_stats* stats) {
+    return run_kaffpa(n, vwgt, xadj, adjcwgt, adjncy, params, edgecut, part, stats);
+}
+
+void kaffpa(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts,
+            double* imbalance, bool suppress_output, int seed, int mode, int* edgecut,
+            int* part) {
+    (void)suppress_output;  // ignored by the reference too (kaHIP_interface.cpp:108-113)
+    (void)mode;
+    kahip_params p;
+    kahip_default_params(&p);
+    p.k = *nparts;
+    p.imbalance = *imbalance;
+    p.seed = seed;
+    if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
+        report("kaffpa", edgecut);
+}
+
+void kaffpa_balance(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts,
+                    double* imbalance, bool perfectly_balance, bool suppress_output, int seed,
+                    int mode, int* edgecut, int* part) {
+    (void)suppress_output;
+    (void)mode;
+    kahip_params p;
+    kahip_default_params(&p);
+    p.k = *nparts;
+    p.imbalance = *imbalance;
+    p.seed = seed;
+    if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr)) {
+        report("kaffpa_balance", edgecut);
+        return;
+    }
+    if (perfectly_balance) {
+        long long W = 0;
+        for (int v = 0; v < *n; ++v) W += vwgt ? vwgt[v] : 1;
+        const long long bound =
+            (long long)((1 + *imbalance) * std::ceil((double)W / (double)*nparts));
+        *edgecut = (int)rebalance(*n, xadj, adjncy, vwgt, adjcwgt, *nparts, bound, part);
+    }
+}
+
+void kaffpa_balance_NE(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts,
+                       double* imbalance, bool suppress_output, int seed, int mode, int* edgecut,
+                       int* part) {
+    (void)suppress_output;
+    (void)mode;
+    // app/balance_configuration.h:27-37: node weight += weighted degree
+    std::vector<int> w(*n);
+    for (int v = 0; v < *n; ++v) {
+        long long s = vwgt ? vwgt[v] : 1;
+        for (int e = xadj[v]; e < xadj[v + 1]; ++e) s += adjcwgt ? adjcwgt[e] : 1;
+        w[v] = (int)s;
+    }
+    kahip_params p;
+    kahip_default_params(&p);
+    p.k = *nparts;
+    p.imbalance = *imbalance;
+    p.seed = seed;
+    if (run_kaffpa(*n, w.data(), xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
+        report("kaffpa_balance_NE", edgecut);
+}
+
+int kahip_graph_from_host(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                          const int* adjwgt, kahip_graph_t* out) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        auto* g = new kahip_graph();
+        hipStream_t s;
+        KH_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            upload_graph(n, xadj, adjncy, vwgt, adjwgt, g->og, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            delete g;
+            throw;
+        }
+        KH_CHECK(hipStreamDestroy(s));
+        *out = g;
+    });
+}
+
+int kahip_graph_rmat(int scale, long long samples, unsigned long long seed, double a, double b,
+                     double c, kahip_graph_t* out) {
+    return guarded([&] {
+        auto* g = new kahip_graph();
+        try {
+            generate_rmat(scale, samples, seed, a, b, c, g->og);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int kahip_graph_info(kahip_graph_t g, int* n, long long* m2, long long* total_vwgt,
+                     int* max_degree) {
+    return guarded([&] {
+        KH_REQUIRE(g, "null graph");
+        if (n) *n = g->og.g.n;
+        if (m2) *m2 = g->og.g.m2;
+        if (total_vwgt) *total_vwgt = g->og.g.total_vwgt;
+        if (max_degree) *max_degree = g->og.g.max_degree;
+    });
+}
+
+int kahip_graph_to_host(kahip_graph_t g, int* xadj, int* adjncy, int* vwgt, int* adjwgt) {
+    return guarded([&] {
+        KH_REQUIRE(g, "null graph");
+        const DGraph& d = g->og.g;
+        if (xadj) KH_CHECK(hipMemcpy(xadj, d.xadj, sizeof(int) * (d.n + 1), hipMemcpyDeviceToHost));
+        if (adjncy && d.m2) KH_CHECK(hipMemcpy(adjncy, d.adjncy, sizeof(int) * d.m2, hipMemcpyDeviceToHost));
+        if (adjwgt && d.m2) KH_CHECK(hipMemcpy(adjwgt, d.adjwgt, sizeof(int) * d.m2, hipMemcpyDeviceToHost));
+        if (vwgt && d.n) KH_CHECK(hipMemcpy(vwgt, d.vwgt, sizeof(int) * d.n, hipMemcpyDeviceToHost));
+    });
+}
+
+void kahip_graph_free(kahip_graph_t g) { delete g; }
+
+int kahip_partition(kahip_graph_t g, const kahip_params* params, int* part, kahip_stats* stats) {
+    return guarded([&] {
+        KH_REQUIRE(g, "null graph");
+        PartitionParams p = to_params(params);
+        Engine eng;
+        std::vector<int> tmp;
+        if (!part) {
+            tmp.resize(std::max(g->og.g.n, 1));
+            part = tmp.data();
+        }
+        PartitionStats st;
+        eng.partition(g->og.g, p, part, &st);
+        if (p.profile) g_last_profile = profile_json(eng.prof);
+        fill_stats(st, stats);
+    });
+}
+
+int kahip_profile_json(char* buf, int len) {
+    if (!buf || len <= 0) return (int)g_last_profile.size() + 1;
+    std::snprintf(buf, (size_t)len, "%s", g_last_profile.c_str());
+    return (int)g_last_profile.size() + 1 <= len ? 0 : (int)g_last_profile.size() + 1;
+}
+
+int kahip_lp_cluster(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                     const int* adjwgt, int bound, int rounds, int subrounds, unsigned seed,
+                     int* cluster, int* num_clusters, int* moves_per_round) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, vwgt, adjwgt, g, s);
+        DevBuf<int> label, size, map;
+        label.reserve(n);
+        size.reserve(n);
+        map.reserve(n);
+        LpWorkspace::init_cluster_labels(g.g, label.p, size.p, s);
+        BinConfig bins;
+        eng.lp.prepare(g.g, n, subrounds, seed, bins, s);
+        eng.lp.run(LpMode::Cluster, g.g, label.p, size.p, bound, rounds, seed, s, nullptr,
+                   "lp_cluster");
+        if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
+        const int nc = eng.cw.remap(g.g, label.p, map.p, s, nullptr);
+        if (n) KH_CHECK(hipMemcpyAsync(cluster, map.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        if (num_clusters) *num_clusters = nc;
+    });
+}
+
+int kahip_remap(int n, const int* label, int* map, int* num_clusters) {
+    return guarded([&] {
+        KH_REQUIRE(n >= 0, "n must be non-negative");
+        for (int v = 0; v < n; ++v) KH_REQUIRE(label[v] >= 0 && label[v] < n, "label out of range");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<int> dl, dm;
+        dl.reserve(n);
+        dm.reserve(n);
+        if (n) KH_CHECK(hipMemcpyAsync(dl.p, label, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        DGraph g;
+        g.n = n;
+        const int nc = eng.cw.remap(g, dl.p, dm.p, s, nullptr);
+        if (n) KH_CHECK(hipMemcpyAsync(map, dm.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        if (num_clusters) *num_clusters = nc;
+    });
+}
+
+int kahip_contract(int n, const int* xadj, const int* adjncy, const int* vwgt, const int* adjwgt,
+                   const int* map, int nc, int* cxadj, int* cadjncy, int* cadjwgt, int* cvwgt) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(nc >= 0 && (n == 0 || nc >= 1), "bad number of clusters");
+        for (int v = 0; v < n; ++v) KH_REQUIRE(map[v] >= 0 && map[v] < nc, "map out of range");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g, c;
+        upload_graph(n, xadj, adjncy, vwgt, adjwgt, g, s);
+        DevBuf<int> dm;
+        dm.reserve(n);
+        if (n) KH_CHECK(hipMemcpyAsync(dm.p, map, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        eng.cw.contract(g.g, dm.p, nc, c, s, nullptr);
+        KH_CHECK(hipMemcpyAsync(cxadj, c.g.xadj, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, s));
+        if (c.g.m2) {
+            KH_CHECK(hipMemcpyAsync(cadjncy, c.g.adjncy, sizeof(int) * c.g.m2, hipMemcpyDeviceToHost, s));
+            KH_CHECK(hipMemcpyAsync(cadjwgt, c.g.adjwgt, sizeof(int) * c.g.m2, hipMemcpyDeviceToHost, s));
+        }
+        if (nc) KH_CHECK(hipMemcpyAsync(cvwgt, c.g.vwgt, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_lp_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, int bound, int rounds, int subrounds, unsigned seed,
+                    int* part, int* moves_per_round) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(k >= 1 && k <= 65536, "k must be in [1, 65536]");
+        for (int v = 0; v < n; ++v) KH_REQUIRE(part[v] >= 0 && part[v] < k, "part out of range");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, vwgt, adjwgt, g, s);
+        DevBuf<int> dp, W;
+        dp.reserve(n);
+        W.reserve(k);
+        if (n) KH_CHECK(hipMemcpyAsync(dp.p, part, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        compute_block_weights(g.g, dp.p, k, W.p, s);
+        BinConfig bins;
+        eng.lp.prepare(g.g, k, subrounds, seed, bins, s);
+        eng.lp.run(LpMode::Refine, g.g, dp.p, W.p, bound, rounds, seed, s, nullptr, "lp_refine");
+        if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
+        if (n) KH_CHECK(hipMemcpyAsync(part, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_edge_cut(int n, const int* xadj, const int* adjncy, const int* adjwgt, const int* part,
+                   long long* cut) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, nullptr, adjwgt, g, s);
+        DevBuf<int> dp;
+        dp.reserve(n);
+        if (n) KH_CHECK(hipMemcpyAsync(dp.p, part, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        *cut = eng.mw.edge_cut(g.g, dp.p, s, nullptr);
+    });
+}
+
+int kahip_block_weights(int n, const int* vwgt, int k, const int* part, long long* weights) {
+    return guarded([&] {
+        KH_REQUIRE(n >= 0 && k >= 1 && k <= 65536, "bad n or k");
+        for (int v = 0; v < n; ++v) KH_REQUIRE(part[v] >= 0 && part[v] < k, "part out of range");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<int> dp, dv, W;
+        dp.reserve(n);
+        dv.reserve(n);
+        W.reserve(k);
+        DGraph g;
+        g.n = n;
+        g.vwgt = dv.p;
+        if (n) {
+            KH_CHECK(hipMemcpyAsync(dp.p, part, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            if (vwgt) {
+                KH_CHECK(hipMemcpyAsync(dv.p, vwgt, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            } else {
+                std::vector<int> ones(n, 1);
+                KH_CHECK(hipMemcpyAsync(dv.p, ones.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+                KH_CHECK(hipStreamSynchronize(s));
+            }
+        }
+        compute_block_weights(g, dp.p, k, W.p, s);
+        std::vector<int> h(k);
+        KH_CHECK(hipMemcpyAsync(h.data(), W.p, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        for (int b = 0; b < k; ++b) weights[b] = h[b];
+    });
+}
+
+int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_part,
+                  int* fine_part) {
+    return guarded([&] {
+        KH_REQUIRE(n_fine >= 0 && n_coarse >= 0, "bad sizes");
+        for (int v = 0; v < n_fine; ++v) KH_REQUIRE(map[v] >= 0 && map[v] < n_coarse, "map out of range");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<int> dm, dc, df;
+        dm.reserve(n_fine);
+        dc.reserve(n_coarse);
+        df.reserve(n_fine);
+        if (n_fine) KH_CHECK(hipMemcpyAsync(dm.p, map, sizeof(int) * n_fine, hipMemcpyHostToDevice, s));
+        if (n_coarse) KH_CHECK(hipMemcpyAsync(dc.p, coarse_part, sizeof(int) * n_coarse, hipMemcpyHostToDevice, s));
+        project_partition(n_fine, dm.p, dc.p, df.p, s, nullptr);
+        if (n_fine) KH_CHECK(hipMemcpyAsync(fine_part, df.p, sizeof(int) * n_fine, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                            const int* adjwgt, int k, long long block_bound, int repetitions,
+                            unsigned long long seed, int* part) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(k >= 1, "k must be >= 1");
+        HostGraph g;
+        g.n = n;
+        g.xadj.assign(xadj, xadj + n + 1);
+        g.adjncy.assign(adjncy, adjncy + xadj[n]);
+        g.adjwgt.resize(xadj[n]);
+        g.vwgt.resize(n);
+        for (int e = 0; e < xadj[n]; ++e) g.adjwgt[e] = adjwgt ? adjwgt[e] : 1;
+        for (int v = 0; v < n; ++v) g.vwgt[v] = vwgt ? vwgt[v] : 1;
+        IpParams p;
+        p.k = k;
+        p.block_bound = block_bound;
+        p.repetitions = repetitions > 0 ? repetitions : ip_repetitions_for(k);
+        p.seed = seed;
+        std::vector<int> out;
+        initial_partition(g, p, out);
+        std::copy(out.begin(), out.end(), part);
+    });
+}
+
+}  // extern "C"

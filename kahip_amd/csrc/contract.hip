@@ -1,0 +1,439 @@
+// Cluster contraction on gfx950: first-appearance relabel (reference:
+// lib/partition/coarsening/clustering/size_constraint_label_propagation.cpp:207-232) and the
+// quotient graph of a clustering (reference: contraction.cpp:94-133 ->
+// lib/partition/uncoarsening/refinement/quotient_graph_refinement/complete_boundary.h:511-562).
+//
+// The reference scans the members of coarse node p in ascending id and their edges in CSR order
+// and emits each neighbouring cluster q at its first appearance. Since the members' CSR rows are
+// contiguous and increasing in node id, that order is the order of min{e : src(e) in p,
+// cluster(adjncy[e]) = q}. The kernels aggregate (sum of weights, min fine edge index) per (p, q)
+// in any thread order and then sort each row by the min edge index: bit-exact by construction.
+//
+//   remap       leader[c] = atomicMin over members; flags of leaders; exclusive scan -> new ids.
+//   count       coarse weights, member counts, summed member degrees (row capacity bound).
+//   rows_lds    one wave per coarse node: LDS hash (q -> w, min e) of <= 512 entries, then an
+//               LDS bitonic sort by min e; rows with more distinct neighbours go to
+//   rows_global one workgroup per coarse node, hash region in global memory, then a segmented
+//               radix sort of the row by min e.
+//   compact     rows are gathered into the final CSR after a scan of the coarse degrees.
+
+#include <cstring>
+#include <vector>
+
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include "common.hpp"
+#include "contract.hpp"
+#include "lp.hpp"
+
+namespace kahip {
+
+namespace {
+
+constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
+constexpr int kRowMaxDistinct = 512;
+
+inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
+
+__global__ void leader_min(int n, const int* __restrict__ label, int* __restrict__ leader) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) atomicMin(&leader[label[v]], v);
+}
+
+__global__ void leader_flag(int n, const int* __restrict__ label, const int* __restrict__ leader,
+                            int* __restrict__ flag) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) flag[v] = leader[label[v]] == v ? 1 : 0;
+}
+
+__global__ void apply_newid(int n, const int* __restrict__ label, const int* __restrict__ leader,
+                            const int* __restrict__ newid, int* __restrict__ map) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) map[v] = newid[leader[label[v]]];
+}
+
+__global__ void count_members(int n, const int* __restrict__ xadj, const int* __restrict__ vwgt,
+                              const int* __restrict__ map, int* __restrict__ cnt,
+                              int* __restrict__ fdeg, int* __restrict__ cvwgt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int p = map[v];
+    atomicAdd(&cnt[p], 1);
+    atomicAdd(&fdeg[p], xadj[v + 1] - xadj[v]);
+    atomicAdd(&cvwgt[p], vwgt[v]);
+}
+
+__global__ void scatter_members(int n, const int* __restrict__ map, const int* __restrict__ moff,
+                                int* __restrict__ cursor, int* __restrict__ members) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int p = map[v];
+    members[moff[p] + atomicAdd(&cursor[p], 1)] = v;
+}
+
+// One wave (= one workgroup) per coarse node.
+__global__ __launch_bounds__(64) void rows_lds(
+    int nc, const int* __restrict__ xadj, const int* __restrict__ adjncy,
+    const int* __restrict__ adjwgt, const int* __restrict__ map, const int* __restrict__ moff,
+    const int* __restrict__ members, const int* __restrict__ fdeg, const int* __restrict__ tpos,
+    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw,
+    int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    __shared__ int K[kRowCap];
+    __shared__ int V[kRowCap];
+    __shared__ int M[kRowCap];
+    __shared__ unsigned long long S[kRowMaxDistinct];
+    __shared__ int used;
+    __shared__ int over;
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (p >= nc) return;  // uniform
+    const int fd = fdeg[p];
+    int cap = 64;
+    while (cap < 2 * fd && cap < kRowCap) cap <<= 1;
+    for (int i = lane; i < cap; i += 64) {
+        K[i] = -1;
+        V[i] = 0;
+        M[i] = 0x7fffffff;
+    }
+    if (lane == 0) {
+        used = 0;
+        over = 0;
+    }
+    __syncthreads();
+    const int mb = moff[p], me = moff[p + 1];
+    for (int base = mb; base < me; base += 64) {
+        const int mi = base + lane;
+        if (mi < me && !over) {
+            const int u = members[mi];
+            for (int e = xadj[u]; e < xadj[u + 1]; ++e) {
+                const int q = map[adjncy[e]];
+                if (q == p) continue;
+                if (over) break;
+                uint32_t h = mix32((uint32_t)q) & (uint32_t)(cap - 1);
+                while (true) {
+                    const int old = atomicCAS(&K[h], -1, q);
+                    if (old == -1) {
+                        if (atomicAdd(&used, 1) >= kRowMaxDistinct) over = 1;
+                    }
+                    if (old == -1 || old == q) {
+                        atomicAdd(&V[h], adjwgt[e]);
+                        atomicMin(&M[h], e);
+                        break;
+                    }
+                    h = (h + 1) & (uint32_t)(cap - 1);
+                }
+            }
+        }
+        __syncthreads();
+        if (over) break;  // uniform after the barrier
+    }
+    __syncthreads();
+    if (over) {
+        if (lane == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
+        return;
+    }
+    // compact (min e, slot) keys, pad to a power of two, bitonic sort ascending
+    __shared__ int nkeys;
+    if (lane == 0) nkeys = 0;
+    __syncthreads();
+    for (int i = lane; i < cap; i += 64) {
+        if (K[i] >= 0) {
+            const int idx = atomicAdd(&nkeys, 1);
+            S[idx] = ((unsigned long long)(uint32_t)M[i] << 32) | (uint32_t)i;
+        }
+    }
+    __syncthreads();
+    const int cnt = nkeys;
+    int len = 1;
+    while (len < cnt) len <<= 1;
+    for (int i = cnt + lane; i < len; i += 64) S[i] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= len; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < len; i += 64) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const unsigned long long a = S[i], b = S[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        S[i] = b;
+                        S[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int off = tpos[p];
+    for (int i = lane; i < cnt; i += 64) {
+        const int slot = (int)(S[i] & 0xffffffffu);
+        tq[off + i] = K[slot];
+        tw[off + i] = V[slot];
+    }
+    if (lane == 0) cdeg[p] = cnt;
+}
+
+// One workgroup per overflowing coarse node; the hash region is pre-cleared and cleared again.
+__global__ __launch_bounds__(256) void rows_global(
+    const int* __restrict__ list, int count, const int* __restrict__ xadj,
+    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
+    const int* __restrict__ moff, const int* __restrict__ members,
+    const long long* __restrict__ hoff, const long long* __restrict__ hcap,
+    const int* __restrict__ obase, int* __restrict__ HK, int* __restrict__ HV,
+    int* __restrict__ HM, int* __restrict__ cdeg, unsigned* __restrict__ okey,
+    unsigned long long* __restrict__ oval) {
+    __shared__ int n_out;
+    const int i = blockIdx.x;
+    if (i >= count) return;
+    const int p = list[i];
+    const long long cap = hcap[i];
+    int* K = HK + hoff[i];
+    int* V = HV + hoff[i];
+    int* Mn = HM + hoff[i];
+    if (threadIdx.x == 0) n_out = 0;
+    const int mb = moff[p], me = moff[p + 1];
+    for (int mi = mb + threadIdx.x; mi < me; mi += 256) {
+        const int u = members[mi];
+        for (int e = xadj[u]; e < xadj[u + 1]; ++e) {
+            const int q = map[adjncy[e]];
+            if (q == p) continue;
+            long long h = (long long)(mix32((uint32_t)q) & (uint32_t)(cap - 1));
+            while (true) {
+                const int old = atomicCAS(&K[h], -1, q);
+                if (old == -1 || old == q) {
+                    atomicAdd(&V[h], adjwgt[e]);
+                    atomicMin(&Mn[h], e);
+                    break;
+                }
+                h = (h + 1) & (cap - 1);
+            }
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    const int base = obase[i];
+    for (long long h = threadIdx.x; h < cap; h += 256) {
+        const int q = K[h];
+        if (q < 0) continue;
+        const int idx = atomicAdd(&n_out, 1);
+        okey[base + idx] = (unsigned)Mn[h];
+        oval[base + idx] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[h];
+        K[h] = -1;
+        V[h] = 0;
+        Mn[h] = 0x7fffffff;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) cdeg[p] = n_out;
+}
+
+__global__ void overflow_ends(const int* __restrict__ list, int count, const int* __restrict__ obase,
+                              const int* __restrict__ cdeg, int* __restrict__ oend) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) oend[i] = obase[i] + cdeg[list[i]];
+}
+
+__global__ void overflow_copyback(const int* __restrict__ list, int count,
+                                  const int* __restrict__ obase, const int* __restrict__ cdeg,
+                                  const int* __restrict__ tpos,
+                                  const unsigned long long* __restrict__ oval_sorted,
+                                  int* __restrict__ tq, int* __restrict__ tw) {
+    const int i = blockIdx.x;
+    if (i >= count) return;
+    const int p = list[i];
+    const int c = cdeg[p];
+    const int src = obase[i], dst = tpos[p];
+    for (int j = threadIdx.x; j < c; j += blockDim.x) {
+        const unsigned long long x = oval_sorted[src + j];
+        tq[dst + j] = (int)(x >> 32);
+        tw[dst + j] = (int)(x & 0xffffffffu);
+    }
+}
+
+__global__ void compact_rows(int nc, const int* __restrict__ tpos, const int* __restrict__ cxadj,
+                             const int* __restrict__ tq, const int* __restrict__ tw,
+                             int* __restrict__ cadj, int* __restrict__ cw) {
+    const int p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= nc) return;
+    const int src = tpos[p], dst = cxadj[p], c = cxadj[p + 1] - dst;
+    for (int j = lane; j < c; j += 64) {
+        cadj[dst + j] = tq[src + j];
+        cw[dst + j] = tw[src + j];
+    }
+}
+
+__global__ void project_kernel(int n, const int* __restrict__ map,
+                               const int* __restrict__ coarse_part, int* __restrict__ fine_part) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) fine_part[v] = coarse_part[map[v]];
+}
+
+__global__ void hub_cap_kernel(const int* __restrict__ list, int count, const int* __restrict__ fdeg,
+                               int nc, long long* __restrict__ cap, int* __restrict__ bound) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int fd = fdeg[list[i]];
+    const long long distinct = fd < nc ? fd : nc;
+    long long c = 64;
+    while (c < 2 * distinct) c <<= 1;
+    cap[i] = c;
+    bound[i] = (int)distinct;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------- host side
+
+void ContractWorkspace::scan_i32(const int* in, int* out, int n, hipStream_t s) {
+    size_t bytes = 0;
+    KH_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, (size_t)n, rocprim::plus<int>(),
+                                     s));
+    char* t = scan_tmp_.reserve((long long)bytes + 1);
+    KH_CHECK(rocprim::exclusive_scan(t, bytes, in, out, 0, (size_t)n, rocprim::plus<int>(), s));
+}
+
+int ContractWorkspace::remap(const DGraph& g, const int* label, int* map, hipStream_t s,
+                             Profiler* prof) {
+    const int n = g.n;
+    if (n == 0) return 0;
+    ProfScope ps(prof, "remap", 16.0 * n, 0, s);
+    leader_.reserve(n);
+    flag_.reserve(n + 1);
+    newid_.reserve(n + 1);
+    fill_async(leader_.p, n, 0x7fffffff, s);
+    hipLaunchKernelGGL(leader_min, dim3(grid_for(n, 256)), dim3(256), 0, s, n, label, leader_.p);
+    hipLaunchKernelGGL(leader_flag, dim3(grid_for(n, 256)), dim3(256), 0, s, n, label, leader_.p,
+                       flag_.p);
+    fill_async(flag_.p + n, 1, 0, s);
+    scan_i32(flag_.p, newid_.p, n + 1, s);
+    hipLaunchKernelGGL(apply_newid, dim3(grid_for(n, 256)), dim3(256), 0, s, n, label, leader_.p,
+                       newid_.p, map);
+    int nc = 0;
+    KH_CHECK(hipMemcpyAsync(&nc, newid_.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    return nc;
+}
+
+void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedGraph& out,
+                                 hipStream_t s, Profiler* prof) {
+    const int n = g.n;
+    out.alloc_nodes(nc);
+    ProfScope ps_all(prof, "contract", 0.0, g.m2, s);
+    cnt_.reserve(nc + 1);
+    fdeg_.reserve(nc + 1);
+    moff_.reserve(nc + 1);
+    tpos_.reserve(nc + 1);
+    cursor_.reserve(nc);
+    cdeg_.reserve(nc + 1);
+    members_.reserve(n);
+    fill_async(cnt_.p, nc + 1, 0, s);
+    fill_async(fdeg_.p, nc + 1, 0, s);
+    fill_async(cursor_.p, nc, 0, s);
+    fill_async(cdeg_.p, nc + 1, 0, s);
+    fill_async(out.g.vwgt, nc, 0, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(count_members, dim3(grid_for(n, 256)), dim3(256), 0, s, n, g.xadj,
+                           g.vwgt, map, cnt_.p, fdeg_.p, out.g.vwgt);
+    }
+    scan_i32(cnt_.p, moff_.p, nc + 1, s);
+    scan_i32(fdeg_.p, tpos_.p, nc + 1, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(scatter_members, dim3(grid_for(n, 256)), dim3(256), 0, s, n, map,
+                           moff_.p, cursor_.p, members_.p);
+    }
+    const long long m2 = g.m2;
+    tq_.reserve(m2);
+    tw_.reserve(m2);
+    ovl_list_.reserve(nc);
+    ovl_count_.reserve(1);
+    fill_async(ovl_count_.p, 1, 0, s);
+    if (nc > 0) {
+        // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
+        // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
+        const double bytes = 12.0 * (double)m2 + 12.0 * n;
+        ProfScope ps(prof, "contract_rows_lds", bytes, m2, s);
+        hipLaunchKernelGGL(rows_lds, dim3(nc), dim3(64), 0, s, nc, g.xadj, g.adjncy, g.adjwgt,
+                           map, moff_.p, members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p,
+                           ovl_list_.p, ovl_count_.p);
+    }
+    int novl = 0;
+    KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    if (novl > 0) {
+        ProfScope ps(prof, "contract_rows_global", 0.0, 0, s);
+        hcap_.reserve(novl);
+        hoff_.reserve(novl);
+        obound_.reserve(novl + 1);
+        obase_.reserve(novl + 1);
+        oend_.reserve(novl);
+        hipLaunchKernelGGL(hub_cap_kernel, dim3(grid_for(novl, 256)), dim3(256), 0, s,
+                           ovl_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p);
+        std::vector<long long> caps(novl), offs(novl);
+        std::vector<int> bounds(novl), bases(novl);
+        KH_CHECK(hipMemcpyAsync(caps.data(), hcap_.p, sizeof(long long) * novl,
+                                hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipMemcpyAsync(bounds.data(), obound_.p, sizeof(int) * novl,
+                                hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        long long htotal = 0, ototal = 0;
+        for (int i = 0; i < novl; ++i) {
+            offs[i] = htotal;
+            htotal += caps[i];
+            bases[i] = (int)ototal;
+            ototal += bounds[i];
+        }
+        KH_CHECK(ototal < (1LL << 31) ? hipSuccess : hipErrorInvalidValue);
+        KH_CHECK(hipMemcpyAsync(hoff_.p, offs.data(), sizeof(long long) * novl,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(obase_.p, bases.data(), sizeof(int) * novl,
+                                hipMemcpyHostToDevice, s));
+        hk_.reserve(htotal);
+        hv_.reserve(htotal);
+        hm_.reserve(htotal);
+        fill_async(hk_.p, htotal, -1, s);
+        fill_async(hv_.p, htotal, 0, s);
+        fill_async(hm_.p, htotal, 0x7fffffff, s);
+        okey_.reserve(ototal);
+        oval_.reserve(ototal);
+        okey2_.reserve(ototal);
+        oval2_.reserve(ototal);
+        hipLaunchKernelGGL(rows_global, dim3(novl), dim3(256), 0, s, ovl_list_.p, novl, g.xadj,
+                           g.adjncy, g.adjwgt, map, moff_.p, members_.p, hoff_.p, hcap_.p,
+                           obase_.p, hk_.p, hv_.p, hm_.p, cdeg_.p, okey_.p, oval_.p);
+        hipLaunchKernelGGL(overflow_ends, dim3(grid_for(novl, 256)), dim3(256), 0, s,
+                           ovl_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
+        size_t bytes = 0;
+        KH_CHECK(rocprim::segmented_radix_sort_pairs(
+            nullptr, bytes, okey_.p, okey2_.p, oval_.p, oval2_.p, (unsigned)ototal,
+            (unsigned)novl, obase_.p, oend_.p, 0, 32, s));
+        char* t = scan_tmp_.reserve((long long)bytes + 1);
+        KH_CHECK(rocprim::segmented_radix_sort_pairs(t, bytes, okey_.p, okey2_.p, oval_.p,
+                                                     oval2_.p, (unsigned)ototal, (unsigned)novl,
+                                                     obase_.p, oend_.p, 0, 32, s));
+        hipLaunchKernelGGL(overflow_copyback, dim3(novl), dim3(256), 0, s, ovl_list_.p, novl,
+                           obase_.p, cdeg_.p, tpos_.p, oval2_.p, tq_.p, tw_.p);
+        KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
+    }
+    scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
+    int mc = 0;
+    KH_CHECK(hipMemcpyAsync(&mc, out.xadj_dev() + nc, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    out.alloc_edges(mc);
+    if (nc > 0) {
+        ProfScope ps(prof, "contract_compact", 16.0 * mc, 0, s);
+        hipLaunchKernelGGL(compact_rows, dim3(grid_for(nc, 4)), dim3(256), 0, s, nc, tpos_.p,
+                           out.g.xadj, tq_.p, tw_.p, out.g.adjncy, out.g.adjwgt);
+    }
+    KH_CHECK(hipGetLastError());
+}
+
+void project_partition(int n_fine, const int* map, const int* coarse_part, int* fine_part,
+                       hipStream_t s, Profiler* prof) {
+    if (n_fine == 0) return;
+    ProfScope ps(prof, "project", 8.0 * n_fine, 0, s);
+    hipLaunchKernelGGL(project_kernel, dim3(grid_for(n_fine, 256)), dim3(256), 0, s, n_fine, map,
+                       coarse_part, fine_part);
+}
+
+}  // namespace kahip

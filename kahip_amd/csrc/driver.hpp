@@ -1,0 +1,89 @@
+// Multilevel driver of the FASTSOCIAL path: GPU LP coarsening + contraction, CPU initial
+// partitioning of the coarsest graph, GPU projection + LP refinement, GPU edge cut.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "contract.hpp"
+#include "devbuf.hpp"
+#include "initial_partition.hpp"
+#include "lp.hpp"
+#include "metrics.hpp"
+
+namespace kahip {
+
+struct PartitionParams {
+    int k = 2;
+    double imbalance = 0.03;
+    int seed = 0;
+    int cluster_rounds = 10;   // app/configuration.h:424 (label_iterations)
+    int refine_rounds = 25;    // app/configuration.h:425 (label_iterations_refinement)
+    int subrounds = 4;
+    int cluster_factor = 18;   // app/configuration.h:422 (cluster_coarsening_factor)
+    int stop_factor = 5000;    // app/configuration.h:497 (num_vert_stop_factor)
+    int ip_repetitions = -1;   // -1: the reference's count (initial_partitioning.cpp:69-81)
+    int ip_threads = 0;
+    BinConfig bins;
+    bool profile = false;
+};
+
+constexpr int kMaxLevels = 64;
+
+struct PartitionStats {
+    long long cut = 0;
+    double balance = 0;
+    unsigned upper_bound = 0;
+    unsigned cluster_bound = 0;
+    int levels = 0;  // number of contractions
+    int level_n[kMaxLevels + 1] = {};
+    long long level_m2[kMaxLevels + 1] = {};
+    double ms_coarsen = 0, ms_initial = 0, ms_refine = 0, ms_cut = 0;
+    long long edges_cluster = 0, edges_contract = 0, edges_refine = 0, edges_cut = 0;
+    long long max_block_weight = 0;
+};
+
+// Per-level seeds of the synchronous schedule.
+inline uint32_t cluster_seed(int seed, int level) {
+    return hash3((uint32_t)seed, 0xC1u, (uint32_t)level);
+}
+inline uint32_t refine_seed(int seed, int level) {
+    return hash3((uint32_t)seed, 0x4Eu, (uint32_t)level);
+}
+
+// app/balance_configuration.h:40-46 with the interface's percent round trip
+// (interface/kaHIP_interface.cpp:115): U = (1 + eps) * ceil(W / k) truncated to unsigned;
+// imbalance == 0 selects the perfectly balanced bound (1 + 0.01) * ceil(W / k) (:41-43).
+unsigned upper_bound_for(long long total_weight, int k, double imbalance);
+// size_constraint_label_propagation.cpp:57: ceil(U / factor).
+unsigned cluster_bound_for(unsigned upper, int factor);
+// initial_partitioning.cpp:69-81 for the FASTSOCIAL preset (16 repetitions, minipreps 4).
+int ip_repetitions_for(int k);
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    // Partitions the device graph g (level 0). part_out: host array of g.n block ids.
+    void partition(const DGraph& g, const PartitionParams& p, int* part_out,
+                   PartitionStats* stats);
+    Profiler prof;
+    hipStream_t stream() const { return s_; }
+
+    // building blocks used by the fine-grained C entry points
+    LpWorkspace lp;
+    ContractWorkspace cw;
+    MetricsWorkspace mw;
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+// Uploads a host CSR (vwgt / adjwgt may be null: unit weights) into an owned device graph.
+void upload_graph(int n, const int* xadj, const int* adjncy, const int* vwgt, const int* adjwgt,
+                  OwnedGraph& out, hipStream_t s);
+void download_graph(const DGraph& g, HostGraph& out, hipStream_t s);
+
+}  // namespace kahip

@@ -1,0 +1,699 @@
+// Synchronous label propagation on gfx950: one kernel family serves both the size-constrained
+// clustering of the coarsening phase (labels = clusters, bound B; reference:
+// lib/partition/coarsening/clustering/size_constraint_label_propagation.cpp:134-194) and the
+// LP refinement of the uncoarsening phase (labels = blocks, bound U; reference:
+// lib/partition/uncoarsening/refinement/label_propagation_refinement/
+// label_propagation_refinement.cpp:21-155).
+//
+// A round is S sub-rounds; a node belongs to sub-round hash3(seed ^ kSubSalt, 0, v) % S. One
+// sub-round is four launches, all order-independent, so the result is bit-identical to the CPU
+// mirror of the same schedule whatever the thread interleaving:
+//   propose  every node of the sub-round picks its best eligible neighbouring label from the
+//            label / weight snapshot; proposers append a mover record and add their weight and
+//            priority key to the target's inc / top (integer atomics: order-free);
+//   filter   (refinement) departures from overloaded blocks;
+//   admit    a proposal is admitted if all proposals into its target fit, or if it is the
+//            target's top-priority proposal and fits alone;
+//   apply    admitted moves update labels and label weights; scratch is reset.
+// Nodes are binned by degree (per sub-round lists built once per call):
+//   deg <= 8/16/32/64   G lanes per node, one edge per lane, shuffle aggregation (no LDS);
+//   deg <= 512          one wave per node, LDS hash table (open addressing, LDS atomics);
+//   larger              one workgroup per node, hash table in a pre-cleared global region.
+
+#include "common.hpp"
+#include "lp.hpp"
+
+namespace kahip {
+
+namespace {
+
+constexpr int kWaveCap = 1024;  // LDS hash capacity per wave (degree <= 512 => load <= 1/2)
+constexpr int kBlock = 256;
+
+struct Cand {
+    int conn;
+    uint32_t tie;
+    int lab;
+    int size;
+};
+
+__device__ inline bool better(int ac, uint32_t at, int al, int bc, uint32_t bt, int bl) {
+    return ac > bc || (ac == bc && (at > bt || (at == bt && al < bl)));
+}
+
+template <int W>
+__device__ inline void reduce_cand(Cand& c) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        int oc = __shfl_xor(c.conn, o, W);
+        uint32_t ot = (uint32_t)__shfl_xor((int)c.tie, o, W);
+        int ol = __shfl_xor(c.lab, o, W);
+        int os = __shfl_xor(c.size, o, W);
+        if (better(oc, ot, ol, c.conn, c.tie, c.lab)) {
+            c.conn = oc;
+            c.tie = ot;
+            c.lab = ol;
+            c.size = os;
+        }
+    }
+}
+
+template <int W>
+__device__ inline int reduce_max(int x) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, W));
+    return x;
+}
+
+// The proposal rule and the scratch updates of one proposing node. Called by one lane.
+template <LpMode M>
+__device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_own,
+                              const Cand& c) {
+    if (c.conn < 0) return;  // no eligible candidate
+    const int size_own = a.size[own];
+    bool go;
+    bool overloaded = false;
+    if (M == LpMode::Cluster) {
+        go = c.conn > conn_own || (c.conn == conn_own && c.size >= size_own);
+    } else {
+        overloaded = size_own > a.bound;
+        go = c.conn > conn_own || overloaded ||
+             (c.conn == conn_own && (long long)c.size + 2LL * w <= (long long)size_own);
+    }
+    if (!go) return;
+    const int gain = c.conn - conn_own;
+    const unsigned long long key = prio_key(gain, v);
+    const int idx = atomicAdd(a.n_movers, 1);
+    a.mv_v[idx] = v;
+    a.mv_t[idx] = c.lab;
+    a.mv_gain[idx] = gain;
+    a.mv_ok[idx] = 1;
+    if (M == LpMode::Refine && overloaded) {
+        atomicAdd(&a.out[own], w);
+        atomicMax(&a.dtop[own], key);
+    } else {
+        atomicAdd(&a.inc[c.lab], w);
+        atomicMax(&a.top[c.lab], key);
+    }
+}
+
+// ---------------------------------------------------------------- G lanes per node (deg <= G)
+template <LpMode M, int G>
+__global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
+                                                        int count) {
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int gi = tid / G;
+    const int gl = threadIdx.x & (G - 1);
+    bool valid = gi < count;
+    const int v = valid ? list[gi] : 0;
+    if (M == LpMode::Refine && valid) valid = a.active[v] != 0;
+    // every lane of the group takes part in the shuffles below: no early exit
+    const int own = valid ? a.label[v] : -1;
+    const int beg = valid ? a.xadj[v] : 0;
+    const int deg = valid ? a.xadj[v + 1] - beg : 0;
+    int lab = -1, w = 0;
+    if (gl < deg) {
+        const int e = beg + gl;
+        lab = a.label[a.adjncy[e]];
+        w = a.adjwgt[e];
+    }
+    int conn = 0;
+    bool leader = lab >= 0;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const int lj = __shfl(lab, j, G);
+        const int wj = __shfl(w, j, G);
+        if (lj == lab) {
+            conn += wj;
+            if (j < gl) leader = false;
+        }
+    }
+    const int conn_own = reduce_max<G>((lab >= 0 && lab == own) ? conn : 0);
+    const int nw = valid ? a.vwgt[v] : 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    if (leader && lab != own) {
+        const int sz = a.size[lab];
+        if ((long long)sz + nw < (long long)a.bound) {
+            c.conn = conn;
+            c.tie = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+            c.lab = lab;
+            c.size = sz;
+        }
+    }
+    reduce_cand<G>(c);
+    if (valid && gl == 0) decide<M>(a, v, own, nw, conn_own, c);
+    if (M == LpMode::Refine) {
+        __shared__ unsigned long long tsum;
+        if (threadIdx.x == 0) tsum = 0;
+        __syncthreads();
+        if (valid && gl == 0) atomicAdd(&tsum, (unsigned long long)deg);
+        __syncthreads();
+        if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[blockIdx.x & 63], tsum);
+    }
+}
+
+// ---------------------------------------------------------------- one wave per node, LDS hash
+template <LpMode M>
+__global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
+                                                       int count) {
+    __shared__ int skey[kBlock / 64][kWaveCap];
+    __shared__ int sval[kBlock / 64][kWaveCap];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * (kBlock / 64) + wave;
+    bool valid = gi < count;
+    const int v = valid ? list[gi] : 0;
+    if (M == LpMode::Refine && valid) valid = a.active[v] != 0;
+    const int own = valid ? a.label[v] : -1;
+    const int beg = valid ? a.xadj[v] : 0;
+    const int deg = valid ? a.xadj[v + 1] - beg : 0;
+    int cap = 64;
+    while (cap < 2 * deg) cap <<= 1;  // deg <= 512 => cap <= kWaveCap
+    int* K = skey[wave];
+    int* V = sval[wave];
+    for (int i = lane; i < cap; i += 64) {
+        K[i] = -1;
+        V[i] = 0;
+    }
+    __syncthreads();
+    for (int e = beg + lane; e < beg + deg; e += 64) {
+        const int lab = a.label[a.adjncy[e]];
+        const int w = a.adjwgt[e];
+        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(cap - 1);
+        while (true) {
+            const int old = atomicCAS(&K[h], -1, lab);
+            if (old == -1 || old == lab) {
+                atomicAdd(&V[h], w);
+                break;
+            }
+            h = (h + 1) & (uint32_t)(cap - 1);
+        }
+    }
+    __syncthreads();
+    const int nw = valid ? a.vwgt[v] : 0;
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (int i = lane; i < cap; i += 64) {
+        const int lab = K[i];
+        if (lab < 0) continue;
+        const int conn = V[i];
+        if (lab == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[lab];
+        if ((long long)sz + nw >= (long long)a.bound) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+    }
+    conn_own = reduce_max<64>(conn_own);
+    reduce_cand<64>(c);
+    if (valid && lane == 0) {
+        decide<M>(a, v, own, nw, conn_own, c);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+    }
+}
+
+// ---------------------------------------------------------------- one workgroup per hub node
+template <LpMode M>
+__global__ __launch_bounds__(kBlock) void propose_block(LpArgs a, const int* __restrict__ list,
+                                                        int count,
+                                                        const long long* __restrict__ tab_off,
+                                                        int* __restrict__ gkey,
+                                                        int* __restrict__ gval) {
+    __shared__ Cand red[kBlock / 64];
+    __shared__ int red_own[kBlock / 64];
+    const int gi = blockIdx.x;
+    if (gi >= count) return;  // uniform per block
+    const int v = list[gi];
+    if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
+    const int own = a.label[v];
+    const int beg = a.xadj[v];
+    const int deg = a.xadj[v + 1] - beg;
+    long long cap = 64;
+    while (cap < 2LL * deg) cap <<= 1;
+    int* K = gkey + tab_off[gi];
+    int* V = gval + tab_off[gi];
+    for (int e = beg + threadIdx.x; e < beg + deg; e += kBlock) {
+        const int lab = a.label[a.adjncy[e]];
+        const int w = a.adjwgt[e];
+        long long h = (long long)(mix32((uint32_t)lab) & (uint32_t)(cap - 1));
+        while (true) {
+            const int old = atomicCAS(&K[h], -1, lab);
+            if (old == -1 || old == lab) {
+                atomicAdd(&V[h], w);
+                break;
+            }
+            h = (h + 1) & (cap - 1);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (long long i = threadIdx.x; i < cap; i += kBlock) {
+        const int lab = K[i];
+        if (lab < 0) continue;
+        const int conn = V[i];
+        K[i] = -1;  // leave the region clean for the next use
+        V[i] = 0;
+        if (lab == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[lab];
+        if ((long long)sz + nw >= (long long)a.bound) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+    }
+    conn_own = reduce_max<64>(conn_own);
+    reduce_cand<64>(c);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave] = c;
+        red_own[wave] = conn_own;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand b = red[0];
+        int co = red_own[0];
+        for (int i = 1; i < kBlock / 64; ++i) {
+            co = max(co, red_own[i]);
+            if (better(red[i].conn, red[i].tie, red[i].lab, b.conn, b.tie, b.lab)) b = red[i];
+        }
+        decide<M>(a, v, own, nw, co, b);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+    }
+}
+
+// ---------------------------------------------------------------- filter / admit / apply
+__global__ void filter_departures(LpArgs a, int max_movers) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= max_movers || i >= *a.n_movers) return;
+    const int v = a.mv_v[i];
+    const int own = a.label[v];
+    const int so = a.size[own];
+    if (so <= a.bound) return;  // not a departure from an overloaded block
+    const int w = a.vwgt[v];
+    const unsigned long long key = prio_key(a.mv_gain[i], v);
+    const bool pass = (long long)so - (long long)a.out[own] >= (long long)a.bound ||
+                      a.dtop[own] == key;
+    if (pass) {
+        const int t = a.mv_t[i];
+        atomicAdd(&a.inc[t], w);
+        atomicMax(&a.top[t], key);
+    } else {
+        a.mv_ok[i] = 0;
+    }
+}
+
+__global__ void admit_moves(LpArgs a, int max_movers) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= max_movers || i >= *a.n_movers) return;
+    if (!a.mv_ok[i]) return;
+    const int v = a.mv_v[i];
+    const int t = a.mv_t[i];
+    const int w = a.vwgt[v];
+    const long long st = a.size[t];
+    const bool ok = st + (long long)a.inc[t] < (long long)a.bound ||
+                    (a.top[t] == prio_key(a.mv_gain[i], v) && st + w < (long long)a.bound);
+    if (!ok) a.mv_ok[i] = 0;
+}
+
+template <LpMode M>
+__global__ void apply_moves(LpArgs a, int max_movers) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= max_movers || i >= *a.n_movers) return;
+    const int v = a.mv_v[i];
+    const int t = a.mv_t[i];
+    const int own = a.label[v];
+    a.inc[t] = 0;
+    a.top[t] = 0ull;
+    if (M == LpMode::Refine) {
+        a.out[own] = 0;
+        a.dtop[own] = 0ull;
+    }
+    if (!a.mv_ok[i]) return;
+    const int w = a.vwgt[v];
+    atomicSub(&a.size[own], w);
+    atomicAdd(&a.size[t], w);
+    a.label[v] = t;
+    atomicAdd(a.moves, 1);
+    if (M == LpMode::Refine) {
+        for (int e = a.xadj[v]; e < a.xadj[v + 1]; ++e) a.next_active[a.adjncy[e]] = 1;
+    }
+}
+
+// ---------------------------------------------------------------- binning
+__device__ inline int bin_of(const BinConfig& cfg, int deg) {
+#pragma unroll
+    for (int b = 0; b < kNumBins - 1; ++b)
+        if (deg <= cfg.hi[b]) return b;
+    return kNumBins - 1;
+}
+
+__global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, int S,
+                          BinConfig cfg, int* __restrict__ counts,
+                          unsigned long long* __restrict__ edges) {
+    __shared__ int local[kMaxSub * kNumBins];
+    __shared__ unsigned long long ledges[kMaxSub * kNumBins];
+    for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) {
+        local[i] = 0;
+        ledges[i] = 0;
+    }
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) {
+        const int deg = xadj[v + 1] - xadj[v];
+        if (deg > 0) {
+            const int l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
+            atomicAdd(&local[l], 1);
+            atomicAdd(&ledges[l], (unsigned long long)deg);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S * kNumBins; i += blockDim.x) {
+        if (local[i]) {
+            atomicAdd(&counts[i], local[i]);
+            atomicAdd(&edges[i], ledges[i]);
+        }
+    }
+}
+
+__global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, int S,
+                            BinConfig cfg, const int* __restrict__ offsets,
+                            int* __restrict__ cursor, int* __restrict__ lists) {
+    __shared__ int local[kMaxSub * kNumBins];
+    __shared__ int base[kMaxSub * kNumBins];
+    for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) local[i] = 0;
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int l = -1, r = 0;
+    if (v < n) {
+        const int deg = xadj[v + 1] - xadj[v];
+        if (deg > 0) {
+            l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
+            r = atomicAdd(&local[l], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S * kNumBins; i += blockDim.x)
+        base[i] = local[i] ? atomicAdd(&cursor[i], local[i]) : 0;
+    __syncthreads();
+    if (l >= 0) lists[offsets[l] + base[l] + r] = v;
+}
+
+__global__ void hub_capacity(const int* __restrict__ list, int count, const int* __restrict__ xadj,
+                             long long* __restrict__ cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int v = list[i];
+    const long long deg = xadj[v + 1] - xadj[v];
+    long long c = 64;
+    while (c < 2 * deg) c <<= 1;
+    cap[i] = c;
+}
+
+__global__ void fill_i32(int* p, long long n, int value) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void fill_u64(unsigned long long* p, long long n, unsigned long long value) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void fill_i64(long long* p, long long n, long long value) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void fill_u8(uint8_t* p, long long n, uint8_t value) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void init_labels(int n, int* __restrict__ label, int* __restrict__ size,
+                            const int* __restrict__ vwgt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) {
+        label[v] = v;
+        size[v] = vwgt[v];
+    }
+}
+
+__global__ void block_weights_kernel(int n, const int* __restrict__ part,
+                                     const int* __restrict__ vwgt, int k, int* __restrict__ W) {
+    extern __shared__ int lw[];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) lw[i] = 0;
+    __syncthreads();
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        atomicAdd(&lw[part[v]], vwgt[v]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x)
+        if (lw[i]) atomicAdd(&W[i], lw[i]);
+}
+
+__global__ void block_weights_global(int n, const int* __restrict__ part,
+                                     const int* __restrict__ vwgt, int* __restrict__ W) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) atomicAdd(&W[part[v]], vwgt[v]);
+}
+
+inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
+
+}  // namespace
+
+// ================================================================= host side
+
+void fill_async(int* p, long long n, int value, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, value);
+}
+void fill_async(unsigned long long* p, long long n, unsigned long long value, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, value);
+}
+void fill_async(uint8_t* p, long long n, uint8_t value, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_u8, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, value);
+}
+void fill_async(long long* p, long long n, long long value, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_i64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, value);
+}
+
+void LpWorkspace::prepare(const DGraph& g, int num_labels, int S, uint32_t seed,
+                          const BinConfig& bins, hipStream_t s) {
+    KH_REQUIRE(S >= 1 && S <= kMaxSub, "sub-rounds must be in [1, 8]");
+    subrounds_ = S;
+    const int L = kMaxSub * kNumBins;
+    lists_.reserve(g.n);
+    counts_.reserve(L);
+    cursor_.reserve(L);
+    offsets_.reserve(L);
+    edges_.reserve(L);
+    mv_v_.reserve(g.n);
+    mv_t_.reserve(g.n);
+    mv_gain_.reserve(g.n);
+    mv_ok_.reserve(g.n);
+    n_movers_.reserve(kMaxSub * kMaxRounds);
+    moves_.reserve(kMaxRounds);
+    inc_.reserve(num_labels);
+    out_.reserve(num_labels);
+    top_.reserve(num_labels);
+    dtop_.reserve(num_labels);
+    active_.reserve(g.n);
+    next_active_.reserve(g.n);
+
+    fill_async(counts_.p, L, 0, s);
+    fill_async(cursor_.p, L, 0, s);
+    fill_async(edges_.p, L, 0ull, s);
+    fill_async(inc_.p, num_labels, 0, s);
+    fill_async(out_.p, num_labels, 0, s);
+    fill_async(top_.p, num_labels, 0ull, s);
+    fill_async(dtop_.p, num_labels, 0ull, s);
+    if (g.n > 0)
+        hipLaunchKernelGGL(bin_count, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
+                           seed, S, bins, counts_.p, edges_.p);
+    KH_CHECK(hipMemcpyAsync(host_counts, counts_.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipMemcpyAsync(host_edges, edges_.p, sizeof(long long) * L, hipMemcpyDeviceToHost,
+                            s));
+    KH_CHECK(hipStreamSynchronize(s));
+    int acc = 0;
+    for (int i = 0; i < L; ++i) {
+        host_offsets[i] = acc;
+        acc += (i < S * kNumBins) ? host_counts[i] : 0;
+    }
+    KH_CHECK(hipMemcpyAsync(offsets_.p, host_offsets, sizeof(int) * L, hipMemcpyHostToDevice, s));
+    if (g.n > 0)
+        hipLaunchKernelGGL(bin_scatter, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
+                           seed, S, bins, offsets_.p, cursor_.p, lists_.p);
+
+    // hub hash tables: one pre-cleared region per hub-list entry, sub-round after sub-round
+    int hubs = 0;
+    for (int sr = 0; sr < S; ++sr) hubs += host_counts[sr * kNumBins + kNumBins - 1];
+    hub_total_ = 0;
+    if (hubs > 0) {
+        hub_caps_.reserve(hubs);
+        hub_off_.reserve(hubs);
+        int pos = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            const int l = sr * kNumBins + kNumBins - 1;
+            const int cnt = host_counts[l];
+            if (cnt == 0) continue;
+            hipLaunchKernelGGL(hub_capacity, dim3(grid_for(cnt, 256)), dim3(256), 0, s,
+                               lists_.p + host_offsets[l], cnt, g.xadj, hub_caps_.p + pos);
+            pos += cnt;
+        }
+        std::vector<long long> caps(hubs);
+        KH_CHECK(hipMemcpyAsync(caps.data(), hub_caps_.p, sizeof(long long) * hubs,
+                                hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        host_hub_off_.assign(hubs, 0);
+        for (int i = 0; i < hubs; ++i) {
+            host_hub_off_[i] = hub_total_;
+            hub_total_ += caps[i];
+        }
+        KH_CHECK(hipMemcpyAsync(hub_off_.p, host_hub_off_.data(), sizeof(long long) * hubs,
+                                hipMemcpyHostToDevice, s));
+        hub_key_.reserve(hub_total_);
+        hub_val_.reserve(hub_total_);
+        fill_async(hub_key_.p, hub_total_, -1, s);
+        fill_async(hub_val_.p, hub_total_, 0, s);
+    }
+    KH_CHECK(hipGetLastError());
+}
+
+static const char* kBinName[kNumBins] = {"g8", "g16", "g32", "g64", "wave", "block"};
+
+void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int bound, int rounds,
+                      uint32_t seed, hipStream_t s, Profiler* prof, const char* phase) {
+    KH_REQUIRE(rounds >= 0 && rounds <= kMaxRounds, "LP rounds must be in [0, 64]");
+    const int S = subrounds_;
+    const bool cl = mode == LpMode::Cluster;
+    fill_async(n_movers_.p, (long long)kMaxSub * kMaxRounds, 0, s);
+    fill_async(moves_.p, kMaxRounds, 0, s);
+    trav_.reserve(64);
+    fill_async(trav_.p, 64, 0ull, s);
+    if (!cl) {
+        fill_async(active_.p, g.n, (uint8_t)1, s);
+        fill_async(next_active_.p, g.n, (uint8_t)0, s);
+    }
+    LpArgs a{};
+    a.xadj = g.xadj;
+    a.adjncy = g.adjncy;
+    a.adjwgt = g.adjwgt;
+    a.vwgt = g.vwgt;
+    a.label = label;
+    a.size = size;
+    a.bound = bound;
+    a.inc = inc_.p;
+    a.top = top_.p;
+    a.out = out_.p;
+    a.dtop = dtop_.p;
+    a.mv_v = mv_v_.p;
+    a.mv_t = mv_t_.p;
+    a.mv_gain = mv_gain_.p;
+    a.mv_ok = mv_ok_.p;
+    a.trav = trav_.p;
+    uint8_t* act = active_.p;
+    uint8_t* nxt = next_active_.p;
+    const std::string base = std::string(phase ? phase : (cl ? "lp_cluster" : "lp_refine"));
+    for (int r = 0; r < rounds; ++r) {
+        a.tie_seed = tie_seed_of(seed, r);
+        a.moves = moves_.p + r;
+        a.active = cl ? nullptr : act;
+        a.next_active = cl ? nullptr : nxt;
+        int hub_pos = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            a.n_movers = n_movers_.p + (r * S + sr);
+            int sub_total = 0;
+            for (int b = 0; b < kNumBins; ++b) {
+                const int l = sr * kNumBins + b;
+                const int cnt = host_counts[l];
+                sub_total += cnt;
+                if (cnt == 0) continue;
+                const int* lst = lists_.p + host_offsets[l];
+                // algorithmic bytes: 12 B per traversed edge (target, weight, label gather) and
+                // 16 B per node (two offsets, weight, own label); label weights are O(moved).
+                const double bytes = cl ? 12.0 * (double)host_edges[l] + 16.0 * cnt : 0.0;
+                ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? host_edges[l] : 0, s);
+                switch (b) {
+                    case 0:
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        break;
+                    case 1:
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        break;
+                    case 2:
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        break;
+                    case 3:
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        break;
+                    case 4:
+                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt);
+                        else hipLaunchKernelGGL((propose_wave<LpMode::Refine>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt);
+                        break;
+                    default:
+                        if (cl) hipLaunchKernelGGL((propose_block<LpMode::Cluster>), dim3(cnt), dim3(kBlock), 0, s, a, lst, cnt, hub_off_.p + hub_pos, hub_key_.p, hub_val_.p);
+                        else hipLaunchKernelGGL((propose_block<LpMode::Refine>), dim3(cnt), dim3(kBlock), 0, s, a, lst, cnt, hub_off_.p + hub_pos, hub_key_.p, hub_val_.p);
+                        hub_pos += cnt;
+                        break;
+                }
+            }
+            if (sub_total == 0) continue;
+            const int gm = grid_for(sub_total, 256);
+            ProfScope ps(prof, base + "_admit_apply", 0.0, 0, s);
+            if (!cl) hipLaunchKernelGGL(filter_departures, dim3(gm), dim3(256), 0, s, a, sub_total);
+            hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_total);
+            if (cl) hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_total);
+            else hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_total);
+        }
+        if (!cl) {
+            std::swap(act, nxt);
+            fill_async(nxt, g.n, (uint8_t)0, s);
+        }
+    }
+    KH_CHECK(hipGetLastError());
+}
+
+void LpWorkspace::read_moves(int rounds, int* out, hipStream_t s) {
+    if (rounds <= 0) return;
+    KH_CHECK(hipMemcpyAsync(out, moves_.p, sizeof(int) * rounds, hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+}
+
+long long LpWorkspace::traversed(hipStream_t s) {
+    unsigned long long h[64];
+    KH_CHECK(hipMemcpyAsync(h, trav_.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    long long t = 0;
+    for (int i = 0; i < 64; ++i) t += (long long)h[i];
+    return t;
+}
+
+void LpWorkspace::init_cluster_labels(const DGraph& g, int* label, int* size, hipStream_t s) {
+    if (g.n > 0)
+        hipLaunchKernelGGL(init_labels, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, label,
+                           size, g.vwgt);
+}
+
+void compute_block_weights(const DGraph& g, const int* part, int k, int* W, hipStream_t s) {
+    KH_REQUIRE(k >= 1 && k <= 65536, "k must be in [1, 65536]");
+    fill_async(W, k, 0, s);
+    if (g.n == 0) return;
+    if (k > 8192) {  // per-workgroup LDS histograms only up to 32 KiB
+        hipLaunchKernelGGL(block_weights_global, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n,
+                           part, g.vwgt, W);
+        return;
+    }
+    const int grid = std::min(grid_for(g.n, 256), 2048);
+    hipLaunchKernelGGL(block_weights_kernel, dim3(grid), dim3(256), sizeof(int) * k, s, g.n,
+                       part, g.vwgt, k, W);
+}
+
+}  // namespace kahip

@@ -1,0 +1,85 @@
+// Host interface of the synchronous LP kernels (clustering and refinement).
+#pragma once
+
+#include "common.hpp"
+#include "devbuf.hpp"
+
+namespace kahip {
+
+constexpr int kMaxSub = 8;
+constexpr int kMaxRounds = 64;
+
+// Kernel arguments shared by the propose / filter / admit / apply launches of one sub-round.
+struct LpArgs {
+    const int* xadj;
+    const int* adjncy;
+    const int* adjwgt;
+    const int* vwgt;
+    int* label;        // label of each node (cluster id or block id)
+    int* size;         // weight of each label
+    int bound;         // admission bound: size + w < bound
+    uint32_t tie_seed;
+    const uint8_t* active;  // refinement frontier (nullptr for clustering)
+    uint8_t* next_active;
+    int* inc;                  // per label: summed weight of proposals into it
+    unsigned long long* top;   // per label: top proposal priority into it
+    int* out;                  // per label: summed weight of departures (refinement)
+    unsigned long long* dtop;  // per label: top departure priority (refinement)
+    int* mv_v;                 // mover records
+    int* mv_t;
+    int* mv_gain;
+    int* mv_ok;
+    int* n_movers;
+    int* moves;
+    unsigned long long* trav;  // refinement: traversed directed edges, 64 spread counters
+};
+
+struct LpStats {
+    int moves[kMaxRounds];
+};
+
+class LpWorkspace {
+public:
+    // Bins the nodes of g into per-(sub-round, degree bin) lists and sizes the scratch for
+    // num_labels labels. One small host readback.
+    void prepare(const DGraph& g, int num_labels, int subrounds, uint32_t seed,
+                 const BinConfig& bins, hipStream_t s);
+    // Runs `rounds` synchronous rounds. label/size are device arrays (node -> label,
+    // label -> weight). Refinement starts from an all-active frontier.
+    void run(LpMode mode, const DGraph& g, int* label, int* size, int bound, int rounds,
+             uint32_t seed, hipStream_t s, Profiler* prof, const char* phase);
+    // label[v] = v, size[v] = vwgt[v]
+    static void init_cluster_labels(const DGraph& g, int* label, int* size, hipStream_t s);
+    // Moves per round of the last run (synchronises the stream).
+    void read_moves(int rounds, int* out, hipStream_t s);
+    // Directed edges traversed by the last refinement run (synchronises the stream).
+    long long traversed(hipStream_t s);
+
+    int host_counts[kMaxSub * kNumBins] = {};
+    long long host_edges[kMaxSub * kNumBins] = {};
+    int host_offsets[kMaxSub * kNumBins] = {};
+
+private:
+    int subrounds_ = 1;
+    DevBuf<int> lists_, counts_, cursor_, offsets_;
+    DevBuf<unsigned long long> edges_;
+    DevBuf<int> mv_v_, mv_t_, mv_gain_, mv_ok_, n_movers_, moves_;
+    DevBuf<int> inc_, out_;
+    DevBuf<unsigned long long> trav_;
+    DevBuf<unsigned long long> top_, dtop_;
+    DevBuf<uint8_t> active_, next_active_;
+    DevBuf<long long> hub_off_, hub_caps_;
+    DevBuf<int> hub_key_, hub_val_;
+    long long hub_total_ = 0;
+    std::vector<long long> host_hub_off_;
+};
+
+void fill_async(int* p, long long n, int value, hipStream_t s);
+void fill_async(unsigned long long* p, long long n, unsigned long long value, hipStream_t s);
+void fill_async(uint8_t* p, long long n, uint8_t value, hipStream_t s);
+void fill_async(long long* p, long long n, long long value, hipStream_t s);
+
+// W[b] = sum of vwgt over nodes in block b (k <= 65536).
+void compute_block_weights(const DGraph& g, const int* part, int k, int* W, hipStream_t s);
+
+}  // namespace kahip

@@ -273,10 +273,13 @@ void orc_project(int n_fine, const int* mapping, const int* coarse_part, int* fi
 // R2: the synchronous schedule. Definitions shared with the GPU kernels:
 //   mix32          Wellons' "lowbias32" integer finaliser.
 //   hash3(s,a,b)   mix32(mix32(mix32(s ^ 0x85ebca6b) ^ a) ^ b)
-//   subround(v,r)  hash3(seed ^ 0x68bc21eb, r, v) % S
+//   subround(v)    hash3(seed ^ 0x68bc21eb, 0, v) % S          (fixed for one call)
 //   tie(v,c,r)     hash3(seed ^ 0x2f6b7d93 ^ (r * 0x9e3779b9), v, c)
-// A candidate c beats d when conn(c) > conn(d), or conn equal and tie(c) > tie(d), or both equal
-// and c < d.
+// Candidate order: c beats d when conn(c) > conn(d), or conn equal and tie(c) > tie(d), or both
+// equal and c < d. Proposal priority: (gain desc, v asc).
+// Admission into a label t within one sub-round: let inc(t) be the summed weight of the
+// proposals into t. If size(t) + inc(t) < bound every proposal is admitted; otherwise only the
+// highest-priority proposal is, and only if size(t) + w < bound.
 // ---------------------------------------------------------------------------------------------
 
 static inline uint32_t mix32(uint32_t x) {
@@ -306,8 +309,16 @@ struct Proposal {
     int w;
 };
 
-// Best candidate over the distinct labels of v's neighbours (excluding own), among those that
-// pass `eligible`. Returns -1 when none. conn_own receives the connection to own label.
+inline bool higher_priority(const Proposal& a, const Proposal& b) {
+    return a.gain > b.gain || (a.gain == b.gain && a.v < b.v);
+}
+
+inline int subround_of(uint32_t seed, int v, int S) {
+    return (int)(orc_hash3(seed ^ kSubSalt, 0u, (uint32_t)v) % (uint32_t)S);
+}
+
+// Best candidate over the distinct labels of v's neighbours other than own that pass
+// `eligible`; -1 when none. conn_own receives the connection to the own label.
 template <class Eligible>
 int best_label(int v, const int* xadj, const int* adjncy, const int* adjwgt, const int* label,
                int own, uint32_t tie_seed, std::vector<int64_t>& acc, std::vector<int>& touched,
@@ -337,31 +348,22 @@ int best_label(int v, const int* xadj, const int* adjncy, const int* adjwgt, con
     return best;
 }
 
-// Sort by (key, gain desc, v asc) — the deterministic admission order.
-void sort_proposals(std::vector<Proposal>& P, bool by_target) {
-    std::sort(P.begin(), P.end(), [&](const Proposal& a, const Proposal& b) {
-        int ka = by_target ? a.to : a.from, kb = by_target ? b.to : b.from;
-        if (ka != kb) return ka < kb;
-        if (a.gain != b.gain) return a.gain > b.gain;
-        return a.v < b.v;
-    });
-}
-
-// Arrival admission: within each target group, admit a proposal iff
-// size[target] + (inclusive prefix of weights in admission order) < bound.
-void admit_arrivals(std::vector<Proposal>& P, const std::vector<int64_t>& size, int64_t bound,
-                    std::vector<Proposal>& out) {
-    sort_proposals(P, true);
-    size_t i = 0;
-    while (i < P.size()) {
-        size_t j = i;
-        int64_t pref = 0;
-        while (j < P.size() && P[j].to == P[i].to) {
-            pref += P[j].w;
-            if (size[P[j].to] + pref < bound) out.push_back(P[j]);
-            ++j;
-        }
-        i = j;
+// The admission rule above, applied to every target group of P; `size` is the snapshot.
+void admit(const std::vector<Proposal>& P, const std::vector<int64_t>& size, int64_t bound,
+           std::vector<Proposal>& out, std::vector<int64_t>& inc, std::vector<int>& top) {
+    for (size_t i = 0; i < P.size(); ++i) {
+        const Proposal& p = P[i];
+        inc[p.to] += p.w;
+        if (top[p.to] < 0 || higher_priority(p, P[top[p.to]])) top[p.to] = (int)i;
+    }
+    for (size_t i = 0; i < P.size(); ++i) {
+        const Proposal& p = P[i];
+        if (size[p.to] + inc[p.to] < bound || (top[p.to] == (int)i && size[p.to] + p.w < bound))
+            out.push_back(p);
+    }
+    for (const Proposal& p : P) {
+        inc[p.to] = 0;
+        top[p.to] = -1;
     }
 }
 
@@ -369,22 +371,23 @@ void admit_arrivals(std::vector<Proposal>& P, const std::vector<int64_t>& size, 
 
 extern "C" {
 
-// Synchronous size-constrained LP clustering (R2). `rounds` rounds of `subrounds` sub-rounds;
-// in sub-round s of round r the nodes with subround(v,r) == s propose, against the labels and
-// cluster weights as they stand at the start of the sub-round: v proposes its best eligible
-// neighbour label c (weight test size[c] + w(v) < bound, as size_constraint_label_propagation.cpp
-// :175) when conn(c) > conn(own). Proposals into one cluster are admitted in (gain desc, v asc)
-// order while size + inclusive prefix < bound; admitted moves are applied together. Labels are
-// then renumbered in first-appearance order. Returns the number of clusters.
+// Synchronous size-constrained LP clustering (R2). `rounds` rounds of `subrounds` sub-rounds; in
+// sub-round s the nodes with subround(v) == s propose against the labels and cluster weights as
+// they stand at the start of the sub-round. v proposes its best neighbour label c with
+// size(c) + w(v) < bound (the weight test of size_constraint_label_propagation.cpp:175) when
+// conn(c) > conn(own), or conn(c) == conn(own) and size(c) >= size(own). Admitted moves are
+// applied together; labels are then renumbered in first-appearance order. Returns the number of
+// clusters.
 int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
                         const int* adjwgt, unsigned bound, int rounds, int subrounds,
                         uint32_t seed, int* cluster_out, int* moves_per_round) {
-    std::vector<int> label(n);
-    std::vector<int64_t> size(n), acc(n, 0);
+    std::vector<int> label(n), sub(n), top(n, -1);
+    std::vector<int64_t> size(n), acc(n, 0), inc(n, 0);
     std::vector<int> touched;
     for (int v = 0; v < n; ++v) {
         label[v] = v;
         size[v] = wv(vwgt, v);
+        sub[v] = subround_of(seed, v, subrounds);
     }
     std::vector<Proposal> P, A;
     const int64_t B = (int64_t)bound;
@@ -395,19 +398,18 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
             P.clear();
             A.clear();
             for (int v = 0; v < n; ++v) {
-                if ((int)(orc_hash3(seed ^ kSubSalt, (uint32_t)r, (uint32_t)v) %
-                          (uint32_t)subrounds) != s)
-                    continue;
+                if (sub[v] != s) continue;
                 int own = label[v];
                 int w = wv(vwgt, v);
                 int64_t conn_own, conn_best;
                 int best = best_label(v, xadj, adjncy, adjwgt, label.data(), own, tie_seed, acc,
                                       touched, &conn_own, &conn_best,
                                       [&](int c) { return size[c] + w < B; });
-                if (best >= 0 && conn_best > conn_own)
+                if (best >= 0 && (conn_best > conn_own ||
+                                  (conn_best == conn_own && size[best] >= size[own])))
                     P.push_back({v, own, best, conn_best - conn_own, w});
             }
-            admit_arrivals(P, size, B, A);
+            admit(P, size, B, A, inc, top);
             for (const Proposal& p : A) {
                 size[p.from] -= p.w;
                 size[p.to] += p.w;
@@ -422,18 +424,22 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
 }
 
 // Synchronous LP refinement (R2). Block weights W. Round 0 has every node active; later rounds
-// the neighbours of nodes moved in the previous round (label_propagation_refinement.cpp:84-92).
-// In sub-round s an active node v with subround(v,r) == s proposes its best neighbouring block b
-// with W[b] + w(v) < bound (:68) when conn(b) > conn(own), or whenever its own block is
-// overloaded (W[own] > bound: own is then not eligible, :68). Departures from an overloaded block
-// are admitted in (gain desc, v asc) order while the block is still overloaded before the move;
-// arrivals are admitted as in the clustering. Admitted moves are applied together.
+// the neighbours of the nodes moved in the previous round (label_propagation_refinement.cpp
+// :84-92). In sub-round s an active node v with subround(v) == s proposes its best neighbouring
+// block b with W(b) + w(v) < bound (:68) when conn(b) > conn(own), when conn(b) == conn(own) and
+// W(b) + 2 w(v) <= W(own), or whenever its own block is overloaded (W(own) > bound, which makes
+// own ineligible at :68). Departures from an overloaded block b all pass when W(b) minus their
+// summed weight stays >= bound; otherwise only the highest-priority departure passes. The
+// surviving proposals are admitted as in the clustering and applied together.
 void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
                         const int* adjwgt, int k, unsigned bound, int rounds, int subrounds,
                         uint32_t seed, int* part, int* moves_per_round) {
-    std::vector<int64_t> W(k, 0), acc(k, 0);
-    std::vector<int> touched;
-    for (int v = 0; v < n; ++v) W[part[v]] += wv(vwgt, v);
+    std::vector<int64_t> W(k, 0), acc(k, 0), inc(k, 0), out(k, 0);
+    std::vector<int> touched, top(k, -1), dtop(k, -1), sub(n);
+    for (int v = 0; v < n; ++v) {
+        W[part[v]] += wv(vwgt, v);
+        sub[v] = subround_of(seed, v, subrounds);
+    }
     std::vector<char> active(n, 1), next(n, 0);
     std::vector<Proposal> P, D, A;
     const int64_t U = (int64_t)bound;
@@ -446,10 +452,7 @@ void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vw
             D.clear();
             A.clear();
             for (int v = 0; v < n; ++v) {
-                if (!active[v]) continue;
-                if ((int)(orc_hash3(seed ^ kSubSalt, (uint32_t)r, (uint32_t)v) %
-                          (uint32_t)subrounds) != s)
-                    continue;
+                if (!active[v] || sub[v] != s) continue;
                 int own = part[v];
                 int w = wv(vwgt, v);
                 int64_t conn_own, conn_best;
@@ -457,23 +460,26 @@ void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vw
                                       &conn_own, &conn_best,
                                       [&](int b) { return W[b] + w < U; });
                 bool overloaded = W[own] > U;
-                if (best >= 0 && (conn_best > conn_own || overloaded))
+                if (best >= 0 && (conn_best > conn_own || overloaded ||
+                                  (conn_best == conn_own && W[best] + 2 * (int64_t)w <= W[own])))
                     P.push_back({v, own, best, conn_best - conn_own, w});
             }
-            // departures from overloaded blocks
-            sort_proposals(P, false);
-            size_t i = 0;
-            while (i < P.size()) {
-                size_t j = i;
-                int64_t pref = 0;
-                while (j < P.size() && P[j].from == P[i].from) {
-                    if (W[P[j].from] <= U || W[P[j].from] - pref > U) D.push_back(P[j]);
-                    pref += P[j].w;
-                    ++j;
-                }
-                i = j;
+            for (size_t i = 0; i < P.size(); ++i) {
+                const Proposal& p = P[i];
+                if (W[p.from] <= U) continue;
+                out[p.from] += p.w;
+                if (dtop[p.from] < 0 || higher_priority(p, P[dtop[p.from]])) dtop[p.from] = (int)i;
             }
-            admit_arrivals(D, W, U, A);
+            for (size_t i = 0; i < P.size(); ++i) {
+                const Proposal& p = P[i];
+                if (W[p.from] <= U || W[p.from] - out[p.from] >= U || dtop[p.from] == (int)i)
+                    D.push_back(p);
+            }
+            for (const Proposal& p : P) {
+                out[p.from] = 0;
+                dtop[p.from] = -1;
+            }
+            admit(D, W, U, A, inc, top);
             for (const Proposal& p : A) {
                 W[p.from] -= p.w;
                 W[p.to] += p.w;

@@ -1,0 +1,234 @@
+"""GPU parity: every HIP step of the path against the oracle, bit for bit, through the C ABI.
+
+The oracle's R2 functions define the synchronous schedule; R1 functions (pinned against the
+reference build and the golden fixtures) define contraction, relabel, cut and projection.
+"""
+import numpy as np
+import pytest
+
+import kahip_amd as K
+import oracle as O
+from conftest import manual_graph, random_graph
+from pipeline_mirror import mirror_partition
+
+pytestmark = pytest.mark.gpu
+
+
+def star(n_leaves, weighted=False):
+    """A hub of degree n_leaves plus a path through the leaves (exercises the hub kernels)."""
+    n = n_leaves + 1
+    edges = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
+    u = np.array([a for a, b in edges] + [b for a, b in edges])
+    v = np.array([b for a, b in edges] + [a for a, b in edges])
+    o = np.lexsort((v, u))
+    u, v = u[o], v[o]
+    xadj = np.zeros(n + 1, np.int64)
+    np.add.at(xadj, u + 1, 1)
+    w = (np.arange(len(u)) % 5 + 1).astype(np.int32) if weighted else None
+    return np.cumsum(xadj).astype(np.int32), v.astype(np.int32), None, w
+
+
+def graphs(rgg15, delaunay15, weighted132):
+    out = {
+        "manual": manual_graph(),
+        "er200w": random_graph(200, 0.05, 1, weighted=True),
+        "er500": random_graph(500, 0.02, 2),
+        "rgg15": rgg15,
+        "delaunay15": delaunay15,
+        "weighted132": weighted132,
+        "star1500": star(1500),
+        "star700w": star(700, weighted=True),
+        "rmat12": K.gen_rmat(12, 16 << 12, 7),
+    }
+    return out
+
+
+@pytest.fixture(scope="module")
+def G(rgg15, delaunay15, weighted132):
+    return graphs(rgg15, delaunay15, weighted132)
+
+
+CLUSTER_CASES = [
+    ("manual", 3, 0), ("er200w", 10, 1), ("er500", 40, 2), ("rgg15", 938, 0), ("rgg15", 16, 3),
+    ("delaunay15", 938, 1), ("weighted132", 40, 4), ("star1500", 200, 5), ("star700w", 2000, 6),
+    ("rmat12", 3751, 7), ("rmat12", 30, 8),
+]
+
+
+@pytest.mark.parametrize("name,bound,seed", CLUSTER_CASES)
+@pytest.mark.parametrize("subrounds", [1, 4])
+def test_lp_cluster_bit_exact(G, name, bound, seed, subrounds):
+    x, a, vw, ew = G[name]
+    got, nc, moves = K.lp_cluster(x, a, vw, ew, bound, 10, subrounds, seed)
+    want, wnc, wmoves = O.lp_cluster_sync(x, a, vw, ew, bound, 10, subrounds, seed)
+    assert nc == wnc
+    assert np.array_equal(moves, wmoves)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", ["manual", "er500", "rgg15", "star1500", "rmat12"])
+def test_remap_bit_exact(G, name):
+    n = len(G[name][0]) - 1
+    rng = np.random.default_rng(n)
+    for labels in (rng.integers(0, max(n // 3, 1), n), np.arange(n)[::-1].copy(),
+                   np.zeros(n, np.int64)):
+        got, nc = K.remap(labels)
+        want, wnc = O.remap_first_appearance(labels)
+        assert nc == wnc and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("name,bound,seed", CLUSTER_CASES)
+def test_contract_bit_exact(G, name, bound, seed):
+    x, a, vw, ew = G[name]
+    cl, nc, _ = O.lp_cluster_sync(x, a, vw, ew, bound, 10, 4, seed)
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+
+
+def test_contract_wide_rows(G):
+    """Coarse rows with more than 512 distinct neighbours take the global-memory path."""
+    x, a, vw, ew = G["star1500"]
+    n = len(x) - 1
+    cl = np.arange(n, dtype=np.int32)
+    cl[1:4] = 0  # hub cluster with three leaves: ~1500 distinct coarse neighbours
+    cl, nc = O.remap_first_appearance(cl)
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+    x, a, vw, ew = G["rmat12"]
+    n = len(x) - 1
+    rng = np.random.default_rng(3)
+    cl, nc = O.remap_first_appearance(rng.integers(0, 40, n))  # a few huge clusters
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+
+
+REFINE_CASES = [("er200w", 2, 0), ("rgg15", 2, 1), ("rgg15", 64, 2), ("delaunay15", 8, 3),
+                ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6)]
+
+
+@pytest.mark.parametrize("name,k,seed", REFINE_CASES)
+@pytest.mark.parametrize("start", ["random", "blocks", "overloaded"])
+def test_lp_refine_bit_exact(G, name, k, seed, start):
+    x, a, vw, ew = G[name]
+    n = len(x) - 1
+    W = n if vw is None else int(vw.sum())
+    U = O.upper_bound(W, k, 0.03)
+    rng = np.random.default_rng(seed)
+    if start == "random":
+        part = rng.integers(0, k, n).astype(np.int32)
+    elif start == "blocks":
+        part = (np.arange(n) * k // max(n, 1)).astype(np.int32)
+    else:
+        part = np.where(rng.random(n) < 0.7, 0, rng.integers(0, k, n)).astype(np.int32)
+    got, moves = K.lp_refine(x, a, vw, ew, k, U, 25, 4, seed, part)
+    want, wmoves = O.lp_refine_sync(x, a, vw, ew, k, U, 25, 4, seed, part)
+    assert np.array_equal(moves, wmoves)
+    assert np.array_equal(got, want)
+    if start != "overloaded":
+        assert O.edge_cut(x, a, ew, got) <= O.edge_cut(x, a, ew, part)
+
+
+@pytest.mark.parametrize("name", ["manual", "er200w", "rgg15", "weighted132", "rmat12"])
+def test_edge_cut_block_weights_project(G, name):
+    x, a, vw, ew = G[name]
+    n = len(x) - 1
+    rng = np.random.default_rng(11)
+    for k in (1, 2, 7, 64):
+        part = rng.integers(0, k, n).astype(np.int32)
+        assert K.edge_cut(x, a, ew, part) == O.edge_cut(x, a, ew, part)
+        assert np.array_equal(K.block_weights(vw, k, part), O.block_weights(vw, k, part))
+    mapping = rng.integers(0, max(n // 4, 1), n).astype(np.int32)
+    cpart = rng.integers(0, 5, max(n // 4, 1)).astype(np.int32)
+    assert np.array_equal(K.project(mapping, cpart), O.project(mapping, cpart))
+
+
+@pytest.mark.parametrize("name,k,seed", [("rgg15", 2, 0), ("rgg15", 8, 1), ("delaunay15", 4, 2),
+                                         ("weighted132", 2, 3), ("rmat12", 8, 4),
+                                         ("manual", 2, 0)])
+def test_pipeline_matches_mirror(G, name, k, seed):
+    x, a, vw, ew = G[name]
+    part, st = K.partition(x, a, vw, ew, k=k, imbalance=0.03, seed=seed)
+    want, info = mirror_partition(x, a, vw, ew, k, 0.03, seed)
+    assert st["levels"] == info["levels"]
+    assert st["level_n"] == info["level_n"]
+    assert np.array_equal(part, want)
+    assert st["cut"] == O.edge_cut(x, a, ew, want)
+    assert st["upper_bound"] == info["U"]
+
+
+def test_pipeline_deterministic(G):
+    x, a, vw, ew = G["rmat12"]
+    p1, s1 = K.partition(x, a, vw, ew, k=16, seed=5)
+    p2, s2 = K.partition(x, a, vw, ew, k=16, seed=5)
+    assert np.array_equal(p1, p2) and s1["cut"] == s2["cut"]
+
+
+@pytest.mark.parametrize("scale,samples,seed", [(10, 16 << 10, 1), (12, 8 << 12, 2)])
+def test_rmat_generator_matches_numpy(scale, samples, seed):
+    g = K.Graph.rmat(scale, samples, seed)
+    x, a, vw, ew = g.to_host()
+    wx, wa, _, ww = K.gen_rmat(scale, samples, seed)
+    assert np.array_equal(x, wx) and np.array_equal(a, wa) and np.array_equal(ew, ww)
+    assert np.all(vw == 1)
+    g.free()
+
+
+def test_device_graph_partition_roundtrip(G):
+    x, a, vw, ew = G["rgg15"]
+    g = K.Graph.from_host(x, a, vw, ew)
+    part, st = g.partition(k=4, seed=3)
+    want, _ = K.partition(x, a, vw, ew, k=4, seed=3)
+    assert np.array_equal(part, want)
+    g.free()
+
+
+def test_kaffpa_dropin_manual_graph():
+    x, a, _, _ = manual_graph()
+    cut, blocks = K.kaffpa(None, x, None, a, 2, 0.03, True, 0, K.FASTSOCIAL)
+    assert cut == O.edge_cut(x, a, None, np.array(blocks, np.int32))
+    assert sorted(set(blocks)) == [0, 1]
+
+
+def test_kaffpa_void_symbol(rgg15):
+    import ctypes as C
+    x, a, _, _ = rgg15
+    n = C.c_int(len(x) - 1)
+    k = C.c_int(2)
+    imb = C.c_double(0.03)
+    cut = C.c_int(0)
+    part = np.zeros(len(x) - 1, np.int32)
+    ip = C.POINTER(C.c_int)
+    K.lib().kaffpa(C.byref(n), None, x.ctypes.data_as(ip), None, a.ctypes.data_as(ip),
+                   C.byref(k), C.byref(imb), True, 0, K.FASTSOCIAL, C.byref(cut),
+                   part.ctypes.data_as(ip))
+    assert cut.value == O.edge_cut(x, a, None, part)
+    assert O.balance(None, 2, part) <= 1.03 + 1e-9
+
+
+@pytest.mark.parametrize("case", ["empty", "edgeless", "k1", "k_gt_n"])
+def test_edge_cases(case):
+    if case == "empty":
+        x, a = np.zeros(1, np.int32), np.zeros(0, np.int32)
+        part, st = K.partition(x, a, k=2)
+        assert len(part) == 0 and st["cut"] == 0
+    elif case == "edgeless":
+        x, a = np.zeros(11, np.int32), np.zeros(0, np.int32)
+        part, st = K.partition(x, a, k=3)
+        assert st["cut"] == 0 and np.bincount(part, minlength=3).max() <= 4
+        cl, nc, _ = K.lp_cluster(x, a, None, None, 5, 10, 4, 0)
+        assert nc == 10 and np.array_equal(cl, np.arange(10))
+    elif case == "k1":
+        x, a, _, _ = manual_graph()
+        part, st = K.partition(x, a, k=1)
+        assert st["cut"] == 0 and np.all(part == 0)
+    else:
+        x, a, _, _ = manual_graph()
+        part, st = K.partition(x, a, k=8)
+        assert part.min() >= 0 and part.max() < 8
+        assert st["cut"] == O.edge_cut(x, a, None, part)
