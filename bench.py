@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the LP coarsening + refinement path on one MI355X.
+
+A step is one full kaffpa(FASTSOCIAL) pass over the workload graph, which is generated on the GPU
+and stays resident in HBM: GPU LP clustering + contraction for every level, CPU initial
+partitioning of the coarsest graph, GPU projection + LP refinement for every level, GPU edge cut.
+
+metric value = GTEPS of the LP path = (directed edges traversed by LP clustering rounds,
+contractions, LP refinement rounds and the cut pass) / (GPU time of coarsening + refinement +
+cut, hipEvents on the library's stream). The CPU initial partitioning is outside the metric; its
+time is in ms_per_step (wall clock per full step) and reported separately.
+
+roofline: the dominant kernel (largest share of the timed LP launches), with algorithmic bytes
+per launch (12 B per traversed edge + 16 B per node) over its hipEvent-measured duration.
+cpu_baseline: the reference's own LP clustering (compiled from its sources, oracle/_ref) timed on
+this host's cores on a bounded sample of the same graph (see the 'sample' field).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "LP coarsen+refine GTEPS & % HBM peak; edge cut learned vs plain at ε=3%"
+
+WORKLOADS = {
+    # R-MAT scale 24, 2^28 samples, (0.57, 0.19, 0.19, 0.05), k = 256, eps = 3 %
+    "rmat24": dict(kind="rmat", scale=24, samples=1 << 28, k=256, seed=1),
+    # smaller variants for quick checks
+    "rmat20": dict(kind="rmat", scale=20, samples=1 << 24, k=256, seed=1),
+    "rmat16": dict(kind="rmat", scale=16, samples=1 << 20, k=64, seed=1),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_graph(K, w):
+    if w["kind"] == "rmat":
+        return K.Graph.rmat(w["scale"], w["samples"], w["seed"])
+    raise ValueError(w["kind"])
+
+
+def algorithmic_bytes(st, cluster_rounds=10):
+    """Algorithmic HBM bytes of the whole path from the per-level stats: LP clustering rounds
+    12 B per traversed edge + 16 B per node; contraction 12 B per fine edge + 16 B per fine node
+    + 8 B per coarse edge and node; LP refinement 12 B per traversed edge; cut 12 B per edge and
+    node."""
+    b = 0.0
+    for lvl in range(st["levels"]):
+        n, m2 = st["level_n"][lvl], st["level_m2"][lvl]
+        nc, mc = st["level_n"][lvl + 1], st["level_m2"][lvl + 1]
+        b += 16.0 * n * cluster_rounds
+        b += 12.0 * m2 + 16.0 * n + 8.0 * mc + 8.0 * nc
+    b += 12.0 * st["edges_cluster"]
+    b += 12.0 * st["edges_refine"]
+    b += 12.0 * st["edges_cut"] + 12.0 * st["level_n"][0]
+    return b
+
+
+def cpu_baseline(K, g, w, budget_s=30.0):
+    """Reference LP (1 of the 10 clustering sweeps) + contraction of its clustering on the full
+    workload graph, single thread, timed inside the reference build."""
+    import oracle as O
+    if not O.ref_available():
+        return None
+    x, a, vw, ew = g.to_host()
+    n = len(x) - 1
+    W = int(vw.sum())
+    U = O.upper_bound(W, w["k"], 0.03)
+    B = O.cluster_bound(U, 18)
+    iters = 1
+    log(f"cpu baseline: reference LP clustering, {iters} sweep(s), bound B={B}, n={n}, 2m={len(a)}")
+    nc, t_lp, t_ct = O.ref_time_lp_level(x, a, None, ew, B, iters, 0)
+    edges = iters * len(a) + len(a)
+    gteps = edges / (t_lp + t_ct) / 1e9
+    return {"value": round(gteps, 5), "unit": "GTEPS", "cores": 1, "kind": "reference",
+            "sample": (f"reference size_constraint_label_propagation, {iters} of 10 sweeps "
+                       f"(bound {B}) + contract_clustering of its {nc} clusters on the full "
+                       f"workload graph (n={n}, 2m={len(a)}), single thread; "
+                       f"lp {t_lp:.2f}s + contraction {t_ct:.2f}s"),
+            "lp_s": round(t_lp, 3), "contract_s": round(t_ct, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="one profiled step, no timing loop (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import kahip_amd as K
+    K.lib()
+    w = WORKLOADS[args.workload]
+    t0 = time.time()
+    g = make_graph(K, w)
+    info = g.info()
+    log(f"graph {args.workload}: n={info['n']} 2m={info['m2']} maxdeg={info['max_degree']} "
+        f"generated in {time.time() - t0:.2f}s")
+
+    if args.profile_only:
+        _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
+        log(json.dumps(st))
+        return
+
+    for i in range(args.warmup):
+        _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False)
+        log(f"warmup {i}: cut={st['cut']} gpu_ms={st['ms_coarsen'] + st['ms_refine'] + st['ms_cut']:.1f} "
+            f"ip_ms={st['ms_initial']:.1f} levels={st['level_n']}")
+    stats = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False)
+        stats.append(st)
+        log(f"step {i}: cut={st['cut']} balance={st['balance']:.4f} "
+            f"coarsen={st['ms_coarsen']:.1f}ms refine={st['ms_refine']:.1f}ms "
+            f"ip={st['ms_initial']:.1f}ms")
+    wall = time.perf_counter() - t_start
+
+    gpu_ms = [s["ms_coarsen"] + s["ms_refine"] + s["ms_cut"] for s in stats]
+    edges = [s["edges_cluster"] + s["edges_contract"] + s["edges_refine"] + s["edges_cut"]
+             for s in stats]
+    gteps = sum(edges) / (sum(gpu_ms) / 1e3) / 1e9
+    path_bytes = algorithmic_bytes(stats[-1])
+    path_gbs = path_bytes / (gpu_ms[-1] / 1e3) / 1e9
+
+    # one profiled step (untimed) for the per-kernel roofline
+    _, pst = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
+    prof = K.profile()
+    timed = {k: v for k, v in prof.items() if v["bytes"] > 0}
+    dom_name = max(timed, key=lambda k: timed[k]["ms"]) if timed else None
+    roofline = None
+    if dom_name:
+        d = timed[dom_name]
+        per_launch_bytes = d["bytes"] / d["launches"]
+        per_launch_ms = d["ms"] / d["launches"]
+        achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": dom_name, "launches": d["launches"],
+                    "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_ms * 1e3, 2)}
+    lp_all = [v for k, v in prof.items() if k.startswith("lp_cluster_propose")]
+    lp_gbs = (sum(v["bytes"] for v in lp_all) / (sum(v["ms"] for v in lp_all) / 1e3) / 1e9
+              if lp_all else None)
+
+    cpu = None
+    if not args.no_cpu_baseline and rank == 0:
+        try:
+            cpu = cpu_baseline(K, g, w)
+        except Exception as e:  # reported, never silently replaced
+            log(f"cpu baseline failed: {e!r}")
+    g.free()
+
+    last = stats[-1]
+    out = {
+        "metric": METRIC,
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": (f"R-MAT scale {w.get('scale')} with {w.get('samples')} samples "
+                                f"(0.57,0.19,0.19,0.05), symmetrised, dedup weight=multiplicity; "
+                                f"kaffpa FASTSOCIAL k={w['k']} eps=0.03"),
+                   "n": info["n"], "m2": info["m2"], "k": w["k"], "imbalance": 0.03,
+                   "parallelism": "single-gpu"},
+        "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
+        "cpu_initial_partition_ms_per_step": round(sum(s["ms_initial"] for s in stats) / len(stats), 1),
+        "path_hbm_gbs": round(path_gbs, 1),
+        "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 4),
+        "lp_cluster_propose_gbs": None if lp_gbs is None else round(lp_gbs, 1),
+        "edge_cut_plain": last["cut"],
+        "edge_cut_learned": None,
+        "balance": round(last["balance"], 5),
+        "levels": last["level_n"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = round(gteps / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -450,6 +450,67 @@ void recurse(const HostGraph& g, int lb, int ub, RecCtx& ctx, std::vector<int>& 
     for (size_t i = 0; i < ids1.size(); ++i) part[ids1[i]] = p1[i];
 }
 
+// Moves nodes out of blocks heavier than `bound` until none is (or no move is possible): nodes
+// of overloaded blocks are ranked by the cut increase of their best move into a block with room
+// (neighbouring blocks, else the lightest block) and moved greedily; rankings are recomputed
+// when a target fills up. Isolated nodes cost nothing and go first.
+void rebalance(const HostGraph& g, int k, long long bound, std::vector<int>& part) {
+    std::vector<long long> W(k, 0);
+    for (int v = 0; v < g.n; ++v) W[part[v]] += g.vwgt[v];
+    std::vector<long long> conn(k, 0);
+    std::vector<int> touched;
+    for (int pass = 0; pass < 64; ++pass) {
+        bool any_over = false;
+        for (int b = 0; b < k; ++b) any_over |= W[b] > bound;
+        if (!any_over) return;
+        const int lightest = (int)(std::min_element(W.begin(), W.end()) - W.begin());
+        struct Move {
+            long long loss;
+            int v, to;
+        };
+        std::vector<Move> moves;
+        for (int v = 0; v < g.n; ++v) {
+            const int own = part[v];
+            if (W[own] <= bound) continue;
+            for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+                const int b = part[g.adjncy[e]];
+                if (conn[b] == 0) touched.push_back(b);
+                conn[b] += g.adjwgt[e];
+            }
+            int t = -1;
+            long long tc = -1;
+            for (int b : touched)
+                if (b != own && W[b] + g.vwgt[v] <= bound && conn[b] > tc) {
+                    t = b;
+                    tc = conn[b];
+                }
+            if (t < 0 && lightest != own && W[lightest] + g.vwgt[v] <= bound) {
+                t = lightest;
+                tc = conn[lightest];
+            }
+            const long long own_conn = conn[own];
+            for (int b : touched) conn[b] = 0;
+            touched.clear();
+            if (t >= 0) moves.push_back({own_conn - tc, v, t});
+        }
+        if (moves.empty()) return;
+        std::sort(moves.begin(), moves.end(), [](const Move& a, const Move& b) {
+            return a.loss < b.loss || (a.loss == b.loss && a.v < b.v);
+        });
+        bool moved = false;
+        for (const Move& m : moves) {
+            const int own = part[m.v];
+            if (W[own] <= bound) continue;
+            if (W[m.to] + g.vwgt[m.v] > bound) continue;  // target filled up: next pass re-ranks
+            W[own] -= g.vwgt[m.v];
+            W[m.to] += g.vwgt[m.v];
+            part[m.v] = m.to;
+            moved = true;
+        }
+        if (!moved) return;
+    }
+}
+
 }  // namespace
 
 void initial_partition(const HostGraph& g, const IpParams& p, std::vector<int>& part) {
@@ -509,6 +570,7 @@ void initial_partition(const HostGraph& g, const IpParams& p, std::vector<int>& 
         part[v] = b;
         heap.emplace(w + g.vwgt[v], b);
     }
+    if (p.block_bound > 0) rebalance(g, k, p.block_bound, part);
 }
 
 }  // namespace kahip

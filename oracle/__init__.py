@@ -114,6 +114,8 @@ def ref():
         L.ref_kaffpa_fastsocial_phases.restype = None
         L.ref_coarsen_one_level.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_double,
                                             C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.ref_time_lp_level.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_uint, C.c_int,
+                                         C.c_int, _f64p]
         L.ref_upper_bound.argtypes = [C.c_int, ip, C.c_int, C.c_double]
         L.ref_upper_bound.restype = C.c_uint
         _ref = L
@@ -290,6 +292,14 @@ def ref_coarsen_one_level(xadj, adjncy, vwgt, adjwgt, k, imbalance, seed):
                                      mapping, cx, ca, cw, cv)
     mc = int(cx[nc])
     return mapping, (cx[:nc + 1].copy(), ca[:mc].copy(), cw[:mc].copy(), cv[:nc].copy())
+
+
+def ref_time_lp_level(xadj, adjncy, vwgt, adjwgt, bound, iterations, seed):
+    """(clusters, lp_seconds, contraction_seconds) of the reference on one graph."""
+    t = np.zeros(2, np.float64)
+    nc = ref().ref_time_lp_level(_n(xadj), xadj, adjncy, _p(vwgt), _p(adjwgt), bound, iterations,
+                                 seed, t)
+    return int(nc), float(t[0]), float(t[1])
 
 
 def ref_upper_bound(n, vwgt, k, imbalance):

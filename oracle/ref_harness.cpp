@@ -280,3 +280,42 @@ REF_API unsigned ref_upper_bound(int n, const int* vwgt, int k, double imbalance
     bc.configurate_balance(cfg, G);
     return cfg.upper_bound_partition;
 }
+
+// Times the reference's LP clustering (size_constraint_label_propagation::label_propagation,
+// size_constraint_label_propagation.cpp:134-194, `iterations` sweeps, bound B) and the
+// contraction of the resulting clustering (contraction.cpp:94-133) on one graph; graph
+// construction is not timed. times[0] = LP seconds, times[1] = contraction seconds. Returns the
+// number of clusters.
+REF_API int ref_time_lp_level(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                              const int* adjwgt, unsigned bound, int iterations, int seed,
+                              double* times) {
+    graph_access G;
+    build_graph(G, n, xadj, adjncy, vwgt, adjwgt);
+    PartitionConfig cfg;
+    fastsocial_config(cfg, 2);
+    cfg.label_iterations = iterations;
+    cfg.graph_allready_partitioned = false;
+    cfg.combine = false;
+    srand(seed);
+    random_functions::setSeed(seed);
+    std::vector<NodeWeight> cid;
+    NodeID blocks = 0;
+    size_constraint_label_propagation sclp;
+    NodeWeight b = bound;
+    double t0 = now_s();
+    sclp.label_propagation(cfg, G, b, cid, blocks);
+    double t1 = now_s();
+    CoarseMapping map(n);
+    for (int v = 0; v < n; ++v) map[v] = cid[v];
+    Matching matching;
+    NodePermutationMap perm;
+    graph_access coarse;
+    contraction c;
+    NodeID ncv = blocks;
+    double t2 = now_s();
+    c.contract_clustering(cfg, G, coarse, matching, map, ncv, perm);
+    double t3 = now_s();
+    times[0] = t1 - t0;
+    times[1] = t3 - t2;
+    return (int)blocks;
+}
