@@ -83,8 +83,9 @@ _lib = None
 
 
 def build(verbose=False):
-    from . import build as _b
-    return _b.build(verbose=verbose)
+    """Compiles libkahip.so for gfx950 in-tree (see build_lib.py)."""
+    from . import build_lib
+    return build_lib.build(verbose=verbose)
 
 
 def lib():
@@ -93,7 +94,7 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise KahipError(f"{LIB_PATH} is missing: run `python -m kahip_amd.build`")
+        raise KahipError(f"{LIB_PATH} is missing: run `python -m kahip_amd.build_lib`")
     L = C.CDLL(LIB_PATH)
     ip = C.POINTER(C.c_int)
     G = C.c_void_p
