@@ -118,6 +118,11 @@ def main():
     if args.profile_only:
         _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
         log(json.dumps(st))
+        prof = K.profile()
+        for name, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
+            gbs = v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 and v["bytes"] > 0 else 0.0
+            log(f"{name:34s} launches={v['launches']:6d} ms={v['ms']:10.2f} "
+                f"GB/s={gbs:8.1f} edges={v['edges']}")
         return
 
     for i in range(args.warmup):

@@ -383,7 +383,7 @@ int kahip_lp_cluster(int n, const int* xadj, const int* adjncy, const int* vwgt,
         map.reserve(n);
         LpWorkspace::init_cluster_labels(g.g, label.p, size.p, s);
         BinConfig bins;
-        eng.lp.prepare(g.g, n, subrounds, seed, bins, s);
+        eng.lp.prepare(LpMode::Cluster, g.g, n, subrounds, seed, bins, s);
         eng.lp.run(LpMode::Cluster, g.g, label.p, size.p, bound, rounds, seed, s, nullptr,
                    "lp_cluster");
         if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
@@ -454,7 +454,7 @@ int kahip_lp_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
         if (n) KH_CHECK(hipMemcpyAsync(dp.p, part, sizeof(int) * n, hipMemcpyHostToDevice, s));
         compute_block_weights(g.g, dp.p, k, W.p, s);
         BinConfig bins;
-        eng.lp.prepare(g.g, k, subrounds, seed, bins, s);
+        eng.lp.prepare(LpMode::Refine, g.g, k, subrounds, seed, bins, s);
         eng.lp.run(LpMode::Refine, g.g, dp.p, W.p, bound, rounds, seed, s, nullptr, "lp_refine");
         if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
         if (n) KH_CHECK(hipMemcpyAsync(part, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));

@@ -33,6 +33,10 @@ namespace {
 
 constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
 constexpr int kRowMaxDistinct = 512;
+constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
+constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
+constexpr int kBigMaxDistinct = 4096;
+constexpr int kHeavyList = 512;    // members whose edges are shared by the whole workgroup
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
 
@@ -174,6 +178,131 @@ __global__ __launch_bounds__(64) void rows_lds(
     if (lane == 0) cdeg[p] = cnt;
 }
 
+// Visits every fine edge of the members mb..me-1 of one coarse node with a whole workgroup:
+// members of degree <= kHeavy are taken one per thread; heavier members are queued in LDS and
+// their edges are then shared by all threads. f(e) is called once per edge; stop() lets the
+// caller abandon the row early.
+template <int THREADS, class F, class Stop>
+__device__ inline void for_member_edges(const int* __restrict__ members, int mb, int me,
+                                        const int* __restrict__ xadj, int* heavy, int* n_heavy,
+                                        F&& f, Stop&& stop) {
+    constexpr int kHeavy = 64;
+    if (threadIdx.x == 0) *n_heavy = 0;
+    __syncthreads();
+    for (int mi = mb + threadIdx.x; mi < me; mi += THREADS) {
+        if (stop()) break;
+        const int u = members[mi];
+        const int b = xadj[u], d = xadj[u + 1] - b;
+        if (d > kHeavy) {
+            const int slot = atomicAdd(n_heavy, 1);
+            if (slot < kHeavyList) {
+                heavy[slot] = u;
+                continue;
+            }
+        }
+        for (int e = b; e < b + d; ++e) f(e);
+    }
+    __syncthreads();
+    const int nh = min(*n_heavy, kHeavyList);
+    for (int h = 0; h < nh; ++h) {
+        if (stop()) break;
+        const int u = heavy[h];
+        for (int e = xadj[u] + threadIdx.x; e < xadj[u + 1]; e += THREADS) f(e);
+    }
+    __syncthreads();
+}
+
+// Rows with more than kRowMaxDistinct distinct neighbours: one workgroup of kBigThreads per row,
+// LDS hash of kBigCap slots and an LDS bitonic sort of up to kBigMaxDistinct entries; rows with
+// even more distinct neighbours are passed on to rows_global.
+__global__ __launch_bounds__(kBigThreads) void rows_lds_big(
+    const int* __restrict__ list, const int* __restrict__ list_count, const int* __restrict__ xadj,
+    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
+    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ tpos,
+    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw,
+    int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    __shared__ int K[kBigCap];
+    __shared__ int V[kBigCap];
+    __shared__ int M[kBigCap];
+    __shared__ unsigned long long S[kBigMaxDistinct];
+    __shared__ int heavy[kHeavyList];
+    __shared__ int n_heavy, used, over, nkeys;
+    const int count = *list_count;
+    for (int li = blockIdx.x; li < count; li += gridDim.x) {
+        const int p = list[li];
+        for (int i = threadIdx.x; i < kBigCap; i += kBigThreads) {
+            K[i] = -1;
+            V[i] = 0;
+            M[i] = 0x7fffffff;
+        }
+        if (threadIdx.x == 0) {
+            used = 0;
+            over = 0;
+            nkeys = 0;
+        }
+        __syncthreads();
+        for_member_edges<kBigThreads>(
+            members, moff[p], moff[p + 1], xadj, heavy, &n_heavy,
+            [&](int e) {
+                if (over) return;
+                const int q = map[adjncy[e]];
+                if (q == p) return;
+                uint32_t h = mix32((uint32_t)q) & (uint32_t)(kBigCap - 1);
+                while (true) {
+                    const int old = atomicCAS(&K[h], -1, q);
+                    if (old == -1 && atomicAdd(&used, 1) >= kBigMaxDistinct) over = 1;
+                    if (old == -1 || old == q) {
+                        atomicAdd(&V[h], adjwgt[e]);
+                        atomicMin(&M[h], e);
+                        break;
+                    }
+                    h = (h + 1) & (uint32_t)(kBigCap - 1);
+                }
+            },
+            [&]() { return over != 0; });
+        if (over) {
+            if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
+            __syncthreads();
+            continue;
+        }
+        for (int i = threadIdx.x; i < kBigCap; i += kBigThreads) {
+            if (K[i] >= 0) {
+                const int idx = atomicAdd(&nkeys, 1);
+                S[idx] = ((unsigned long long)(uint32_t)M[i] << 32) | (uint32_t)i;
+            }
+        }
+        __syncthreads();
+        const int cnt = nkeys;
+        int len = 1;
+        while (len < cnt) len <<= 1;
+        for (int i = cnt + threadIdx.x; i < len; i += kBigThreads) S[i] = ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= len; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < len; i += kBigThreads) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const unsigned long long a = S[i], b = S[l];
+                        if ((a > b) == ((i & k) == 0)) {
+                            S[i] = b;
+                            S[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const int off = tpos[p];
+        for (int i = threadIdx.x; i < cnt; i += kBigThreads) {
+            const int slot = (int)(S[i] & 0xffffffffu);
+            tq[off + i] = K[slot];
+            tw[off + i] = V[slot];
+        }
+        if (threadIdx.x == 0) cdeg[p] = cnt;
+        __syncthreads();
+    }
+}
+
 // One workgroup per overflowing coarse node; the hash region is pre-cleared and cleared again.
 __global__ __launch_bounds__(256) void rows_global(
     const int* __restrict__ list, int count, const int* __restrict__ xadj,
@@ -192,12 +321,13 @@ __global__ __launch_bounds__(256) void rows_global(
     int* V = HV + hoff[i];
     int* Mn = HM + hoff[i];
     if (threadIdx.x == 0) n_out = 0;
-    const int mb = moff[p], me = moff[p + 1];
-    for (int mi = mb + threadIdx.x; mi < me; mi += 256) {
-        const int u = members[mi];
-        for (int e = xadj[u]; e < xadj[u + 1]; ++e) {
+    __shared__ int heavy[kHeavyList];
+    __shared__ int n_heavy;
+    for_member_edges<256>(
+        members, moff[p], moff[p + 1], xadj, heavy, &n_heavy,
+        [&](int e) {
             const int q = map[adjncy[e]];
-            if (q == p) continue;
+            if (q == p) return;
             long long h = (long long)(mix32((uint32_t)q) & (uint32_t)(cap - 1));
             while (true) {
                 const int old = atomicCAS(&K[h], -1, q);
@@ -208,8 +338,8 @@ __global__ __launch_bounds__(256) void rows_global(
                 }
                 h = (h + 1) & (cap - 1);
             }
-        }
-    }
+        },
+        []() { return false; });
     __threadfence();
     __syncthreads();
     const int base = obase[i];
@@ -346,8 +476,9 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     tq_.reserve(m2);
     tw_.reserve(m2);
     ovl_list_.reserve(nc);
-    ovl_count_.reserve(1);
-    fill_async(ovl_count_.p, 1, 0, s);
+    ovl_count_.reserve(2);
+    ovl2_list_.reserve(nc);
+    fill_async(ovl_count_.p, 2, 0, s);
     if (nc > 0) {
         // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
@@ -357,8 +488,14 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
                            map, moff_.p, members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p,
                            ovl_list_.p, ovl_count_.p);
     }
+    if (nc > 0) {
+        ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);
+        hipLaunchKernelGGL(rows_lds_big, dim3(256), dim3(kBigThreads), 0, s, ovl_list_.p,
+                           ovl_count_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
+                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1);
+    }
     int novl = 0;
-    KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
     if (novl > 0) {
         ProfScope ps(prof, "contract_rows_global", 0.0, 0, s);
@@ -368,7 +505,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         obase_.reserve(novl + 1);
         oend_.reserve(novl);
         hipLaunchKernelGGL(hub_cap_kernel, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p);
+                           ovl2_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p);
         std::vector<long long> caps(novl), offs(novl);
         std::vector<int> bounds(novl), bases(novl);
         KH_CHECK(hipMemcpyAsync(caps.data(), hcap_.p, sizeof(long long) * novl,
@@ -383,7 +520,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             bases[i] = (int)ototal;
             ototal += bounds[i];
         }
-        KH_CHECK(ototal < (1LL << 31) ? hipSuccess : hipErrorInvalidValue);
+        KH_REQUIRE(ototal < (1LL << 31), "contraction scratch exceeds 2^31 entries");
         KH_CHECK(hipMemcpyAsync(hoff_.p, offs.data(), sizeof(long long) * novl,
                                 hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(obase_.p, bases.data(), sizeof(int) * novl,
@@ -398,11 +535,11 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         oval_.reserve(ototal);
         okey2_.reserve(ototal);
         oval2_.reserve(ototal);
-        hipLaunchKernelGGL(rows_global, dim3(novl), dim3(256), 0, s, ovl_list_.p, novl, g.xadj,
+        hipLaunchKernelGGL(rows_global, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl, g.xadj,
                            g.adjncy, g.adjwgt, map, moff_.p, members_.p, hoff_.p, hcap_.p,
                            obase_.p, hk_.p, hv_.p, hm_.p, cdeg_.p, okey_.p, oval_.p);
         hipLaunchKernelGGL(overflow_ends, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
+                           ovl2_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
         size_t bytes = 0;
         KH_CHECK(rocprim::segmented_radix_sort_pairs(
             nullptr, bytes, okey_.p, okey2_.p, oval_.p, oval2_.p, (unsigned)ototal,
@@ -411,7 +548,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         KH_CHECK(rocprim::segmented_radix_sort_pairs(t, bytes, okey_.p, okey2_.p, oval_.p,
                                                      oval2_.p, (unsigned)ototal, (unsigned)novl,
                                                      obase_.p, oend_.p, 0, 32, s));
-        hipLaunchKernelGGL(overflow_copyback, dim3(novl), dim3(256), 0, s, ovl_list_.p, novl,
+        hipLaunchKernelGGL(overflow_copyback, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl,
                            obase_.p, cdeg_.p, tpos_.p, oval2_.p, tq_.p, tw_.p);
         KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
     }

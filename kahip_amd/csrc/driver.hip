@@ -153,7 +153,7 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
         size.reserve(fine.n);
         LpWorkspace::init_cluster_labels(fine, label.p, size.p, s);
         const uint32_t cseed = cluster_seed(p.seed, level);
-        lp.prepare(fine, fine.n, p.subrounds, cseed, p.bins, s);
+        lp.prepare(LpMode::Cluster, fine, fine.n, p.subrounds, cseed, p.bins, s);
         lp.run(LpMode::Cluster, fine, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
                "lp_cluster");
         long long edges_round = 0;
@@ -218,7 +218,7 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
         }
         compute_block_weights(gl, part_l, p.k, W.p, s);
         const uint32_t rseed = refine_seed(p.seed, l);
-        lp.prepare(gl, p.k, p.subrounds, rseed, p.bins, s);
+        lp.prepare(LpMode::Refine, gl, p.k, p.subrounds, rseed, p.bins, s);
         lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, prof,
                "lp_refine");
         S.edges_refine += lp.traversed(s);

@@ -27,8 +27,15 @@ namespace kahip {
 
 namespace {
 
-constexpr int kWaveCap = 1024;  // LDS hash capacity per wave (degree <= 512 => load <= 1/2)
+constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => load <= 1/2)
+constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
 constexpr int kBlock = 256;
+constexpr int kHubChunk = 4096;   // hub edges per workgroup
+constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
+constexpr int kHubThreads = 512;
+constexpr int kHubDense = 4096;   // dense LDS counters per hub (refinement, k <= 4096)
+constexpr int kHubDenseThreads = 1024;
+constexpr int kBigDegree = 64;    // moved nodes above this degree mark neighbours per workgroup
 
 struct Cand {
     int conn;
@@ -152,12 +159,14 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     }
 }
 
-// ---------------------------------------------------------------- one wave per node, LDS hash
-template <LpMode M>
+// ---------------------------------------------------------------- one wave per node
+// Labels are aggregated in LDS: a hash table (open addressing, LDS atomics) in general, or a
+// dense counter per label when the labels are the k blocks of the refinement and k is small.
+template <LpMode M, bool DENSE>
 __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
-                                                       int count) {
-    __shared__ int skey[kBlock / 64][kWaveCap];
-    __shared__ int sval[kBlock / 64][kWaveCap];
+                                                       int count, int num_labels) {
+    __shared__ int skey[kBlock / 64][DENSE ? 1 : kWaveCap];
+    __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : kWaveCap];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int gi = blockIdx.x * (kBlock / 64) + wave;
@@ -168,17 +177,25 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     const int beg = valid ? a.xadj[v] : 0;
     const int deg = valid ? a.xadj[v + 1] - beg : 0;
     int cap = 64;
-    while (cap < 2 * deg) cap <<= 1;  // deg <= 512 => cap <= kWaveCap
+    if (DENSE) {
+        cap = num_labels;
+    } else {
+        while (cap < 2 * deg) cap <<= 1;  // deg <= 512 => cap <= kWaveCap
+    }
     int* K = skey[wave];
     int* V = sval[wave];
     for (int i = lane; i < cap; i += 64) {
-        K[i] = -1;
+        if (!DENSE) K[i] = -1;
         V[i] = 0;
     }
     __syncthreads();
     for (int e = beg + lane; e < beg + deg; e += 64) {
         const int lab = a.label[a.adjncy[e]];
         const int w = a.adjwgt[e];
+        if (DENSE) {
+            atomicAdd(&V[lab], w);
+            continue;
+        }
         uint32_t h = mix32((uint32_t)lab) & (uint32_t)(cap - 1);
         while (true) {
             const int old = atomicCAS(&K[h], -1, lab);
@@ -193,18 +210,20 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     const int nw = valid ? a.vwgt[v] : 0;
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (int i = lane; i < cap; i += 64) {
-        const int lab = K[i];
-        if (lab < 0) continue;
-        const int conn = V[i];
-        if (lab == own) {
-            conn_own = conn;
-            continue;
+    if (valid) {
+        for (int i = lane; i < cap; i += 64) {
+            const int lab = DENSE ? (V[i] ? i : -1) : K[i];
+            if (lab < 0) continue;
+            const int conn = V[i];
+            if (lab == own) {
+                conn_own = conn;
+                continue;
+            }
+            const int sz = a.size[lab];
+            if ((long long)sz + nw >= (long long)a.bound) continue;
+            const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+            if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
         }
-        const int sz = a.size[lab];
-        if ((long long)sz + nw >= (long long)a.bound) continue;
-        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
     }
     conn_own = reduce_max<64>(conn_own);
     reduce_cand<64>(c);
@@ -214,59 +233,11 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     }
 }
 
-// ---------------------------------------------------------------- one workgroup per hub node
-template <LpMode M>
-__global__ __launch_bounds__(kBlock) void propose_block(LpArgs a, const int* __restrict__ list,
-                                                        int count,
-                                                        const long long* __restrict__ tab_off,
-                                                        int* __restrict__ gkey,
-                                                        int* __restrict__ gval) {
-    __shared__ Cand red[kBlock / 64];
-    __shared__ int red_own[kBlock / 64];
-    const int gi = blockIdx.x;
-    if (gi >= count) return;  // uniform per block
-    const int v = list[gi];
-    if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
-    const int own = a.label[v];
-    const int beg = a.xadj[v];
-    const int deg = a.xadj[v + 1] - beg;
-    long long cap = 64;
-    while (cap < 2LL * deg) cap <<= 1;
-    int* K = gkey + tab_off[gi];
-    int* V = gval + tab_off[gi];
-    for (int e = beg + threadIdx.x; e < beg + deg; e += kBlock) {
-        const int lab = a.label[a.adjncy[e]];
-        const int w = a.adjwgt[e];
-        long long h = (long long)(mix32((uint32_t)lab) & (uint32_t)(cap - 1));
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, lab);
-            if (old == -1 || old == lab) {
-                atomicAdd(&V[h], w);
-                break;
-            }
-            h = (h + 1) & (cap - 1);
-        }
-    }
-    __threadfence();
-    __syncthreads();
-    const int nw = a.vwgt[v];
-    int conn_own = 0;
-    Cand c{-1, 0u, 0x7fffffff, 0};
-    for (long long i = threadIdx.x; i < cap; i += kBlock) {
-        const int lab = K[i];
-        if (lab < 0) continue;
-        const int conn = V[i];
-        K[i] = -1;  // leave the region clean for the next use
-        V[i] = 0;
-        if (lab == own) {
-            conn_own = conn;
-            continue;
-        }
-        const int sz = a.size[lab];
-        if ((long long)sz + nw >= (long long)a.bound) continue;
-        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
-    }
+// Workgroup-wide reduction of (conn_own, best candidate); the result is valid in thread 0.
+template <int THREADS>
+__device__ inline void block_reduce(int& conn_own, Cand& c) {
+    __shared__ Cand red[THREADS / 64];
+    __shared__ int red_own[THREADS / 64];
     conn_own = reduce_max<64>(conn_own);
     reduce_cand<64>(c);
     const int wave = threadIdx.x >> 6;
@@ -276,14 +247,186 @@ __global__ __launch_bounds__(kBlock) void propose_block(LpArgs a, const int* __r
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        Cand b = red[0];
-        int co = red_own[0];
-        for (int i = 1; i < kBlock / 64; ++i) {
-            co = max(co, red_own[i]);
-            if (better(red[i].conn, red[i].tie, red[i].lab, b.conn, b.tie, b.lab)) b = red[i];
+        for (int i = 1; i < THREADS / 64; ++i) {
+            conn_own = max(conn_own, red_own[i]);
+            if (better(red[i].conn, red[i].tie, red[i].lab, c.conn, c.tie, c.lab)) c = red[i];
         }
-        decide<M>(a, v, own, nw, co, b);
-        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+    }
+}
+
+// ---------------------------------------------------------------- hubs, refinement: dense LDS
+// One workgroup per hub; one LDS counter per block (k <= kHubDense).
+__global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
+    LpArgs a, const int* __restrict__ list, int count, int num_labels) {
+    __shared__ int cnt[kHubDense];
+    const int gi = blockIdx.x;
+    if (gi >= count) return;  // uniform per block
+    const int v = list[gi];
+    if (a.active[v] == 0) return;  // uniform per block
+    const int own = a.label[v];
+    const int beg = a.xadj[v];
+    const int deg = a.xadj[v + 1] - beg;
+    for (int i = threadIdx.x; i < num_labels; i += kHubDenseThreads) cnt[i] = 0;
+    __syncthreads();
+    for (int e = beg + threadIdx.x; e < beg + deg; e += kHubDenseThreads)
+        atomicAdd(&cnt[a.label[a.adjncy[e]]], a.adjwgt[e]);
+    __syncthreads();
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
+        const int conn = cnt[b];
+        if (conn == 0) continue;
+        if (b == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[b];
+        if ((long long)sz + nw >= (long long)a.bound) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
+        if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
+    }
+    block_reduce<kHubDenseThreads>(conn_own, c);
+    if (threadIdx.x == 0) {
+        decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
+        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+    }
+}
+
+// ---------------------------------------------------------------- hubs: chunked LDS hash
+// One workgroup per chunk of kHubChunk edges of a hub. The chunk's labels are aggregated in an
+// LDS hash table; a hub of one chunk decides right away, otherwise the partial sums are added
+// into the hub's global table (one atomic pair per distinct label of the chunk) and
+// hub_finalize decides.
+template <LpMode M>
+__global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
+    LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    const long long* __restrict__ tab_off, const long long* __restrict__ tab_cap,
+    int* __restrict__ gkey, int* __restrict__ gval) {
+    __shared__ int K[kHubCap];
+    __shared__ int V[kHubCap];
+    __shared__ int hub;
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+        }
+        hub = lo;
+    }
+    __syncthreads();
+    const int gi = hub;
+    const int v = list[gi];
+    if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
+    const int nchunks = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int c_idx = chunk - chunk_prefix[gi];
+    const int beg = a.xadj[v];
+    const int end = a.xadj[v + 1];
+    const int e0 = beg + c_idx * kHubChunk;
+    const int e1 = min(end, e0 + kHubChunk);
+    for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
+        K[i] = -1;
+        V[i] = 0;
+    }
+    __syncthreads();
+    for (int e = e0 + threadIdx.x; e < e1; e += kHubThreads) {
+        const int lab = a.label[a.adjncy[e]];
+        const int w = a.adjwgt[e];
+        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(kHubCap - 1);
+        while (true) {
+            const int old = atomicCAS(&K[h], -1, lab);
+            if (old == -1 || old == lab) {
+                atomicAdd(&V[h], w);
+                break;
+            }
+            h = (h + 1) & (uint32_t)(kHubCap - 1);
+        }
+    }
+    __syncthreads();
+    if (nchunks > 1) {
+        int* GK = gkey + tab_off[gi];
+        int* GV = gval + tab_off[gi];
+        const long long cap = tab_cap[gi];
+        for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
+            const int lab = K[i];
+            if (lab < 0) continue;
+            long long h = (long long)(mix32((uint32_t)lab) & (uint32_t)(cap - 1));
+            while (true) {
+                const int old = atomicCAS(&GK[h], -1, lab);
+                if (old == -1 || old == lab) {
+                    atomicAdd(&GV[h], V[i]);
+                    break;
+                }
+                h = (h + 1) & (cap - 1);
+            }
+        }
+        return;
+    }
+    const int own = a.label[v];
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
+        const int lab = K[i];
+        if (lab < 0) continue;
+        const int conn = V[i];
+        if (lab == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[lab];
+        if ((long long)sz + nw >= (long long)a.bound) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+    }
+    block_reduce<kHubThreads>(conn_own, c);
+    if (threadIdx.x == 0) {
+        decide<M>(a, v, own, nw, conn_own, c);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(end - beg));
+    }
+}
+
+// One workgroup per multi-chunk hub: reads (and clears) the hub's global table and decides.
+template <LpMode M>
+__global__ __launch_bounds__(256) void hub_finalize(LpArgs a, const int* __restrict__ list,
+                                                    const int* __restrict__ multi, int n_multi,
+                                                    const long long* __restrict__ tab_off,
+                                                    const long long* __restrict__ tab_cap,
+                                                    int* __restrict__ gkey,
+                                                    int* __restrict__ gval) {
+    const int mi = blockIdx.x;
+    if (mi >= n_multi) return;
+    const int gi = multi[mi];
+    const int v = list[gi];
+    if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
+    int* GK = gkey + tab_off[gi];
+    int* GV = gval + tab_off[gi];
+    const long long cap = tab_cap[gi];
+    const int own = a.label[v];
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (long long i = threadIdx.x; i < cap; i += 256) {
+        const int lab = GK[i];
+        if (lab < 0) continue;
+        const int conn = GV[i];
+        GK[i] = -1;  // leave the region clean for the next use
+        GV[i] = 0;
+        if (lab == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[lab];
+        if ((long long)sz + nw >= (long long)a.bound) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+    }
+    block_reduce<256>(conn_own, c);
+    if (threadIdx.x == 0) {
+        decide<M>(a, v, own, nw, conn_own, c);
+        if (M == LpMode::Refine)
+            atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
     }
 }
 
@@ -341,7 +484,23 @@ __global__ void apply_moves(LpArgs a, int max_movers) {
     a.label[v] = t;
     atomicAdd(a.moves, 1);
     if (M == LpMode::Refine) {
-        for (int e = a.xadj[v]; e < a.xadj[v + 1]; ++e) a.next_active[a.adjncy[e]] = 1;
+        const int beg = a.xadj[v], end = a.xadj[v + 1];
+        if (end - beg > kBigDegree) {
+            a.big[atomicAdd(a.n_big, 1)] = v;
+        } else {
+            for (int e = beg; e < end; ++e) a.next_active[a.adjncy[e]] = 1;
+        }
+    }
+}
+
+// Frontier marking for moved high-degree nodes: grid-stride over the big movers, one workgroup
+// per node at a time.
+__global__ __launch_bounds__(256) void mark_big_neighbours(LpArgs a) {
+    const int nb = *a.n_big;
+    for (int i = blockIdx.x; i < nb; i += gridDim.x) {
+        const int v = a.big[i];
+        for (int e = a.xadj[v] + threadIdx.x; e < a.xadj[v + 1]; e += 256)
+            a.next_active[a.adjncy[e]] = 1;
     }
 }
 
@@ -404,15 +563,10 @@ __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, 
     if (l >= 0) lists[offsets[l] + base[l] + r] = v;
 }
 
-__global__ void hub_capacity(const int* __restrict__ list, int count, const int* __restrict__ xadj,
-                             long long* __restrict__ cap) {
+__global__ void hub_degrees(const int* __restrict__ list, int count, const int* __restrict__ xadj,
+                            int* __restrict__ deg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int v = list[i];
-    const long long deg = xadj[v + 1] - xadj[v];
-    long long c = 64;
-    while (c < 2 * deg) c <<= 1;
-    cap[i] = c;
+    if (i < count) deg[i] = xadj[list[i] + 1] - xadj[list[i]];
 }
 
 __global__ void fill_i32(int* p, long long n, int value) {
@@ -481,10 +635,14 @@ void fill_async(long long* p, long long n, long long value, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(fill_i64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, value);
 }
 
-void LpWorkspace::prepare(const DGraph& g, int num_labels, int S, uint32_t seed,
+void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, uint32_t seed,
                           const BinConfig& bins, hipStream_t s) {
     KH_REQUIRE(S >= 1 && S <= kMaxSub, "sub-rounds must be in [1, 8]");
+    KH_REQUIRE(bins.hi[kNumBins - 2] <= 512, "the wave bin holds degrees <= 512");
     subrounds_ = S;
+    num_labels_ = num_labels;
+    dense_wave_ = mode == LpMode::Refine && num_labels <= kWaveDense;
+    dense_hub_ = mode == LpMode::Refine && num_labels <= kHubDense;
     const int L = kMaxSub * kNumBins;
     lists_.reserve(g.n);
     counts_.reserve(L);
@@ -503,6 +661,8 @@ void LpWorkspace::prepare(const DGraph& g, int num_labels, int S, uint32_t seed,
     dtop_.reserve(num_labels);
     active_.reserve(g.n);
     next_active_.reserve(g.n);
+    big_.reserve(g.n);
+    n_big_.reserve(kMaxSub * kMaxRounds);
 
     fill_async(counts_.p, L, 0, s);
     fill_async(cursor_.p, L, 0, s);
@@ -528,49 +688,87 @@ void LpWorkspace::prepare(const DGraph& g, int num_labels, int S, uint32_t seed,
         hipLaunchKernelGGL(bin_scatter, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
                            seed, S, bins, offsets_.p, cursor_.p, lists_.p);
 
-    // hub hash tables: one pre-cleared region per hub-list entry, sub-round after sub-round
+    // hubs (last bin): chunk prefix per sub-round list, global tables for multi-chunk hubs
     int hubs = 0;
-    for (int sr = 0; sr < S; ++sr) hubs += host_counts[sr * kNumBins + kNumBins - 1];
+    for (int sr = 0; sr < S; ++sr) {
+        hub_base_[sr] = hubs;
+        hub_n_[sr] = host_counts[sr * kNumBins + kNumBins - 1];
+        hubs += hub_n_[sr];
+    }
     hub_total_ = 0;
-    if (hubs > 0) {
-        hub_caps_.reserve(hubs);
-        hub_off_.reserve(hubs);
-        int pos = 0;
-        for (int sr = 0; sr < S; ++sr) {
-            const int l = sr * kNumBins + kNumBins - 1;
-            const int cnt = host_counts[l];
-            if (cnt == 0) continue;
-            hipLaunchKernelGGL(hub_capacity, dim3(grid_for(cnt, 256)), dim3(256), 0, s,
-                               lists_.p + host_offsets[l], cnt, g.xadj, hub_caps_.p + pos);
-            pos += cnt;
+    for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = 0;
+    if (hubs == 0 || dense_hub_) {
+        KH_CHECK(hipGetLastError());
+        return;
+    }
+    hub_deg_.reserve(hubs);
+    for (int sr = 0; sr < S; ++sr) {
+        const int l = sr * kNumBins + kNumBins - 1;
+        if (hub_n_[sr] == 0) continue;
+        hipLaunchKernelGGL(hub_degrees, dim3(grid_for(hub_n_[sr], 256)), dim3(256), 0, s,
+                           lists_.p + host_offsets[l], hub_n_[sr], g.xadj,
+                           hub_deg_.p + hub_base_[sr]);
+    }
+    std::vector<int> deg(hubs);
+    KH_CHECK(hipMemcpyAsync(deg.data(), hub_deg_.p, sizeof(int) * hubs, hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    std::vector<int> prefix, multi;
+    std::vector<long long> toff(hubs, 0), tcap(hubs, 0);
+    for (int sr = 0; sr < S; ++sr) {
+        chunk_base_[sr] = (int)prefix.size();
+        multi_base_[sr] = (int)multi.size();
+        int c = 0;
+        for (int i = 0; i < hub_n_[sr]; ++i) {
+            const int d = deg[hub_base_[sr] + i];
+            const int nch = (d + kHubChunk - 1) / kHubChunk;
+            prefix.push_back(c);
+            c += nch;
+            if (nch > 1) {
+                long long cap = 64;
+                while (cap < 2LL * d) cap <<= 1;
+                toff[hub_base_[sr] + i] = hub_total_;
+                tcap[hub_base_[sr] + i] = cap;
+                hub_total_ += cap;
+                multi.push_back(i);
+            }
         }
-        std::vector<long long> caps(hubs);
-        KH_CHECK(hipMemcpyAsync(caps.data(), hub_caps_.p, sizeof(long long) * hubs,
-                                hipMemcpyDeviceToHost, s));
-        KH_CHECK(hipStreamSynchronize(s));
-        host_hub_off_.assign(hubs, 0);
-        for (int i = 0; i < hubs; ++i) {
-            host_hub_off_[i] = hub_total_;
-            hub_total_ += caps[i];
-        }
-        KH_CHECK(hipMemcpyAsync(hub_off_.p, host_hub_off_.data(), sizeof(long long) * hubs,
+        prefix.push_back(c);
+        hub_chunks_[sr] = c;
+        hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
+    }
+    chunk_prefix_.reserve((long long)prefix.size());
+    tab_off_.reserve(hubs);
+    tab_cap_.reserve(hubs);
+    multi_.reserve((long long)multi.size());
+    KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
+                            hipMemcpyHostToDevice, s));
+    KH_CHECK(hipMemcpyAsync(tab_off_.p, toff.data(), sizeof(long long) * hubs,
+                            hipMemcpyHostToDevice, s));
+    KH_CHECK(hipMemcpyAsync(tab_cap_.p, tcap.data(), sizeof(long long) * hubs,
+                            hipMemcpyHostToDevice, s));
+    if (!multi.empty())
+        KH_CHECK(hipMemcpyAsync(multi_.p, multi.data(), sizeof(int) * multi.size(),
                                 hipMemcpyHostToDevice, s));
+    if (hub_total_ > 0) {
         hub_key_.reserve(hub_total_);
         hub_val_.reserve(hub_total_);
         fill_async(hub_key_.p, hub_total_, -1, s);
         fill_async(hub_val_.p, hub_total_, 0, s);
     }
+    KH_CHECK(hipStreamSynchronize(s));  // the host vectors above must outlive the copies
     KH_CHECK(hipGetLastError());
 }
 
-static const char* kBinName[kNumBins] = {"g8", "g16", "g32", "g64", "wave", "block"};
+static const char* kBinName[kNumBins] = {"g8", "g16", "g32", "g64", "wave", "hub"};
 
 void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int bound, int rounds,
                       uint32_t seed, hipStream_t s, Profiler* prof, const char* phase) {
     KH_REQUIRE(rounds >= 0 && rounds <= kMaxRounds, "LP rounds must be in [0, 64]");
     const int S = subrounds_;
     const bool cl = mode == LpMode::Cluster;
+    KH_REQUIRE(cl || !(dense_wave_ || dense_hub_) || num_labels_ > 0, "prepare() not called");
     fill_async(n_movers_.p, (long long)kMaxSub * kMaxRounds, 0, s);
+    fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
     trav_.reserve(64);
     fill_async(trav_.p, 64, 0ull, s);
@@ -595,17 +793,19 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.mv_gain = mv_gain_.p;
     a.mv_ok = mv_ok_.p;
     a.trav = trav_.p;
+    a.big = big_.p;
     uint8_t* act = active_.p;
     uint8_t* nxt = next_active_.p;
     const std::string base = std::string(phase ? phase : (cl ? "lp_cluster" : "lp_refine"));
+    const int nl = num_labels_;
     for (int r = 0; r < rounds; ++r) {
         a.tie_seed = tie_seed_of(seed, r);
         a.moves = moves_.p + r;
         a.active = cl ? nullptr : act;
         a.next_active = cl ? nullptr : nxt;
-        int hub_pos = 0;
         for (int sr = 0; sr < S; ++sr) {
             a.n_movers = n_movers_.p + (r * S + sr);
+            a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
             for (int b = 0; b < kNumBins; ++b) {
                 const int l = sr * kNumBins + b;
@@ -635,14 +835,27 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
                         break;
                     case 4:
-                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt);
-                        else hipLaunchKernelGGL((propose_wave<LpMode::Refine>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt);
+                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
+                        else if (dense_wave_) hipLaunchKernelGGL((propose_wave<LpMode::Refine, true>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
+                        else hipLaunchKernelGGL((propose_wave<LpMode::Refine, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
                         break;
-                    default:
-                        if (cl) hipLaunchKernelGGL((propose_block<LpMode::Cluster>), dim3(cnt), dim3(kBlock), 0, s, a, lst, cnt, hub_off_.p + hub_pos, hub_key_.p, hub_val_.p);
-                        else hipLaunchKernelGGL((propose_block<LpMode::Refine>), dim3(cnt), dim3(kBlock), 0, s, a, lst, cnt, hub_off_.p + hub_pos, hub_key_.p, hub_val_.p);
-                        hub_pos += cnt;
+                    default: {
+                        if (!cl && dense_hub_) {
+                            hipLaunchKernelGGL(propose_hub_dense, dim3(cnt), dim3(kHubDenseThreads), 0, s, a, lst, cnt, nl);
+                            break;
+                        }
+                        const int* cp = chunk_prefix_.p + chunk_base_[sr];
+                        const long long* to = tab_off_.p + hub_base_[sr];
+                        const long long* tc = tab_cap_.p + hub_base_[sr];
+                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, to, tc, hub_key_.p, hub_val_.p);
+                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, to, tc, hub_key_.p, hub_val_.p);
+                        if (hub_multi_n_[sr] > 0) {
+                            const int* ml = multi_.p + multi_base_[sr];
+                            if (cl) hipLaunchKernelGGL((hub_finalize<LpMode::Cluster>), dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, ml, hub_multi_n_[sr], to, tc, hub_key_.p, hub_val_.p);
+                            else hipLaunchKernelGGL((hub_finalize<LpMode::Refine>), dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, ml, hub_multi_n_[sr], to, tc, hub_key_.p, hub_val_.p);
+                        }
                         break;
+                    }
                 }
             }
             if (sub_total == 0) continue;
@@ -650,8 +863,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
             ProfScope ps(prof, base + "_admit_apply", 0.0, 0, s);
             if (!cl) hipLaunchKernelGGL(filter_departures, dim3(gm), dim3(256), 0, s, a, sub_total);
             hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_total);
-            if (cl) hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_total);
-            else hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_total);
+            if (cl) {
+                hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_total);
+            } else {
+                hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_total);
+                hipLaunchKernelGGL(mark_big_neighbours, dim3(std::min(gm, 1024)), dim3(256), 0, s, a);
+            }
         }
         if (!cl) {
             std::swap(act, nxt);

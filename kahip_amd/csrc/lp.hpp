@@ -31,6 +31,8 @@ struct LpArgs {
     int* mv_ok;
     int* n_movers;
     int* moves;
+    int* big;                  // refinement: moved nodes of degree > kBigDegree
+    int* n_big;
     unsigned long long* trav;  // refinement: traversed directed edges, 64 spread counters
 };
 
@@ -42,7 +44,7 @@ class LpWorkspace {
 public:
     // Bins the nodes of g into per-(sub-round, degree bin) lists and sizes the scratch for
     // num_labels labels. One small host readback.
-    void prepare(const DGraph& g, int num_labels, int subrounds, uint32_t seed,
+    void prepare(LpMode mode, const DGraph& g, int num_labels, int subrounds, uint32_t seed,
                  const BinConfig& bins, hipStream_t s);
     // Runs `rounds` synchronous rounds. label/size are device arrays (node -> label,
     // label -> weight). Refinement starts from an all-active frontier.
@@ -61,6 +63,17 @@ public:
 
 private:
     int subrounds_ = 1;
+    int num_labels_ = 0;
+    bool dense_wave_ = false, dense_hub_ = false;
+    // per sub-round hub bookkeeping (hub list = last degree bin)
+    int hub_n_[kMaxSub] = {};
+    int hub_chunks_[kMaxSub] = {};
+    int hub_multi_n_[kMaxSub] = {};
+    int hub_base_[kMaxSub] = {};    // offset into tab_off_/tab_cap_ (list order)
+    int chunk_base_[kMaxSub] = {};  // offset into chunk_prefix_
+    int multi_base_[kMaxSub] = {};  // offset into multi_
+    DevBuf<int> hub_deg_, chunk_prefix_, multi_, big_, n_big_;
+    DevBuf<long long> tab_off_, tab_cap_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
     DevBuf<unsigned long long> edges_;
     DevBuf<int> mv_v_, mv_t_, mv_gain_, mv_ok_, n_movers_, moves_;
@@ -68,10 +81,8 @@ private:
     DevBuf<unsigned long long> trav_;
     DevBuf<unsigned long long> top_, dtop_;
     DevBuf<uint8_t> active_, next_active_;
-    DevBuf<long long> hub_off_, hub_caps_;
     DevBuf<int> hub_key_, hub_val_;
     long long hub_total_ = 0;
-    std::vector<long long> host_hub_off_;
 };
 
 void fill_async(int* p, long long n, int value, hipStream_t s);

@@ -39,6 +39,7 @@ def graphs(rgg15, delaunay15, weighted132):
         "star1500": star(1500),
         "star700w": star(700, weighted=True),
         "rmat12": K.gen_rmat(12, 16 << 12, 7),
+        "star10k": star(10000),
     }
     return out
 
@@ -51,7 +52,7 @@ def G(rgg15, delaunay15, weighted132):
 CLUSTER_CASES = [
     ("manual", 3, 0), ("er200w", 10, 1), ("er500", 40, 2), ("rgg15", 938, 0), ("rgg15", 16, 3),
     ("delaunay15", 938, 1), ("weighted132", 40, 4), ("star1500", 200, 5), ("star700w", 2000, 6),
-    ("rmat12", 3751, 7), ("rmat12", 30, 8),
+    ("rmat12", 3751, 7), ("rmat12", 30, 8), ("star10k", 300, 9), ("star10k", 20000, 10),
 ]
 
 
@@ -98,6 +99,15 @@ def test_contract_wide_rows(G):
     want = O.contract(x, a, vw, ew, cl, nc)
     for g_, w_ in zip(got, want):
         assert np.array_equal(g_, w_)
+    x, a, vw, ew = G["star10k"]
+    n = len(x) - 1
+    cl = np.arange(n, dtype=np.int32)
+    cl[1:4] = 0  # ~10000 distinct coarse neighbours: beyond the LDS tiers
+    cl, nc = O.remap_first_appearance(cl)
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
     x, a, vw, ew = G["rmat12"]
     n = len(x) - 1
     rng = np.random.default_rng(3)
@@ -109,7 +119,8 @@ def test_contract_wide_rows(G):
 
 
 REFINE_CASES = [("er200w", 2, 0), ("rgg15", 2, 1), ("rgg15", 64, 2), ("delaunay15", 8, 3),
-                ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6)]
+                ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6), ("star10k", 4, 7),
+                ("star10k", 5000, 8), ("rmat12", 3000, 9)]
 
 
 @pytest.mark.parametrize("name,k,seed", REFINE_CASES)
