@@ -17,10 +17,10 @@ struct IpParams {
     int k = 2;
     long long block_bound = 0;  // U: every final block must weigh at most this
     int repetitions = 2;        // independent recursive-bisection runs; the lowest cut wins
-    int bisection_tries = 4;    // BFS-grown starts per bisection (reference: bipartition_tries)
+    int bisection_tries = 32;   // BFS-grown starts per bisection, on its coarsest (<= stop_nodes) graph
     int coarsen_factor = 12;    // cluster bound of the bisection coarsening: ceil(max side / 12)
-    int stop_nodes = 200;       // bisection coarsening stops below this many nodes
-    int fm_stall = 100;         // FM stops after this many moves without improvement
+    int stop_nodes = 400;       // bisection coarsening stops below this many nodes
+    int fm_stall = 500;         // FM stops after this many moves without improvement
     uint64_t seed = 0;
     int threads = 0;            // 0: hardware concurrency (capped at 16)
 };
