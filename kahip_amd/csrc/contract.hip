@@ -11,10 +11,11 @@
 //
 //   remap       leader[c] = atomicMin over members; flags of leaders; exclusive scan -> new ids.
 //   count       coarse weights, member counts, summed member degrees (row capacity bound).
-//   rows_lds    one wave per coarse node: LDS hash (q -> w, min e) of <= 512 entries, then an
-//               LDS bitonic sort by min e; rows with more distinct neighbours go to
-//   rows_global one workgroup per coarse node, hash region in global memory, then a segmented
-//               radix sort of the row by min e.
+//   rows_lds      one wave per coarse node: LDS hash (q -> w, min e) of <= 512 entries, then an
+//                 LDS bitonic sort by min e; rows with more distinct neighbours go to
+//   rows_lds_big  one 1024-thread workgroup per row, 128 KB of LDS, <= 4096 distinct; beyond that
+//   rows_global_* the row's member edges in chunks of 4096 over many workgroups, LDS
+//                 pre-aggregation, one global table per row, then a segmented radix sort by min e.
 //   compact     rows are gathered into the final CSR after a scan of the coarse degrees.
 
 #include <cstring>
@@ -37,6 +38,9 @@ constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
 constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
 constexpr int kBigMaxDistinct = 4096;
 constexpr int kHeavyList = 512;    // members whose edges are shared by the whole workgroup
+constexpr int kGChunk = 4096;      // member edges per workgroup in the global tier
+constexpr int kGCap = 8192;        // LDS hash slots of a global-tier chunk
+constexpr int kGThreads = 512;
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
 
@@ -214,7 +218,7 @@ __device__ inline void for_member_edges(const int* __restrict__ members, int mb,
 
 // Rows with more than kRowMaxDistinct distinct neighbours: one workgroup of kBigThreads per row,
 // LDS hash of kBigCap slots and an LDS bitonic sort of up to kBigMaxDistinct entries; rows with
-// even more distinct neighbours are passed on to rows_global.
+// even more distinct neighbours are passed on to the global tier.
 __global__ __launch_bounds__(kBigThreads) void rows_lds_big(
     const int* __restrict__ list, const int* __restrict__ list_count, const int* __restrict__ xadj,
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
@@ -303,15 +307,113 @@ __global__ __launch_bounds__(kBigThreads) void rows_lds_big(
     }
 }
 
-// One workgroup per overflowing coarse node; the hash region is pre-cleared and cleared again.
-__global__ __launch_bounds__(256) void rows_global(
-    const int* __restrict__ list, int count, const int* __restrict__ xadj,
-    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
-    const int* __restrict__ moff, const int* __restrict__ members,
-    const long long* __restrict__ hoff, const long long* __restrict__ hcap,
-    const int* __restrict__ obase, int* __restrict__ HK, int* __restrict__ HV,
-    int* __restrict__ HM, int* __restrict__ cdeg, unsigned* __restrict__ okey,
-    unsigned long long* __restrict__ oval) {
+__global__ void member_degrees(int n, const int* __restrict__ members,
+                               const int* __restrict__ xadj, int* __restrict__ mdeg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mdeg[i] = xadj[members[i] + 1] - xadj[members[i]];
+}
+
+// Rows beyond the LDS tiers, split into chunks of kGChunk member edges (the row's member edges
+// flattened through the member-degree prefix mpre, which equals tpos at the row's first member).
+// Each chunk aggregates (q -> sum w, min e) in LDS and adds its entries into the row's global
+// table, one atomic triple per distinct neighbour of the chunk.
+__global__ __launch_bounds__(kGThreads) void rows_global_chunks(
+    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    const int* __restrict__ xadj, const int* __restrict__ adjncy, const int* __restrict__ adjwgt,
+    const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
+    const int* __restrict__ mpre, const long long* __restrict__ hoff,
+    const long long* __restrict__ hcap, int* __restrict__ HK, int* __restrict__ HV,
+    int* __restrict__ HM) {
+    __shared__ int K[kGCap];
+    __shared__ int V[kGCap];
+    __shared__ int M[kGCap];
+    __shared__ int row;
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = count;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+        }
+        row = lo;
+    }
+    for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
+        K[i] = -1;
+        V[i] = 0;
+        M[i] = 0x7fffffff;
+    }
+    __syncthreads();
+    const int ri = row;
+    const int p = list[ri];
+    const int mb = moff[p], me = moff[p + 1];
+    const long long f_row = mpre[mb];
+    const long long f_end = mpre[me];
+    const long long f0 = f_row + (long long)(chunk - chunk_prefix[ri]) * kGChunk;
+    const long long f1 = min(f_end, f0 + kGChunk);
+    // each thread walks a contiguous run of the chunk
+    const long long per = (kGChunk + kGThreads - 1) / kGThreads;
+    long long a0 = f0 + threadIdx.x * per;
+    const long long a1 = min(f1, a0 + per);
+    if (a0 < a1) {
+        int lo = mb, hi = me;  // largest member index with mpre <= a0
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((long long)mpre[mid] <= a0) lo = mid; else hi = mid;
+        }
+        int mi = lo;
+        int u = members[mi];
+        long long mstart = mpre[mi];
+        long long mend = mstart + (xadj[u + 1] - xadj[u]);
+        for (long long f = a0; f < a1; ++f) {
+            while (f >= mend) {
+                ++mi;
+                u = members[mi];
+                mstart = mpre[mi];
+                mend = mstart + (xadj[u + 1] - xadj[u]);
+            }
+            const int e = xadj[u] + (int)(f - mstart);
+            const int q = map[adjncy[e]];
+            if (q == p) continue;
+            uint32_t h = mix32((uint32_t)q) & (uint32_t)(kGCap - 1);
+            while (true) {
+                const int old = atomicCAS(&K[h], -1, q);
+                if (old == -1 || old == q) {
+                    atomicAdd(&V[h], adjwgt[e]);
+                    atomicMin(&M[h], e);
+                    break;
+                }
+                h = (h + 1) & (uint32_t)(kGCap - 1);
+            }
+        }
+    }
+    __syncthreads();
+    const long long cap = hcap[ri];
+    int* GK = HK + hoff[ri];
+    int* GV = HV + hoff[ri];
+    int* GM = HM + hoff[ri];
+    for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
+        const int q = K[i];
+        if (q < 0) continue;
+        long long h = (long long)(mix32((uint32_t)q) & (uint32_t)(cap - 1));
+        while (true) {
+            const int old = atomicCAS(&GK[h], -1, q);
+            if (old == -1 || old == q) {
+                atomicAdd(&GV[h], V[i]);
+                atomicMin(&GM[h], M[i]);
+                break;
+            }
+            h = (h + 1) & (cap - 1);
+        }
+    }
+}
+
+// One workgroup per row beyond the LDS tiers: moves the row's table entries to the sort buffers
+// (key = min fine edge, value = (q, w)) and clears the table.
+__global__ __launch_bounds__(256) void rows_global_extract(
+    const int* __restrict__ list, int count, const long long* __restrict__ hoff,
+    const long long* __restrict__ hcap, const int* __restrict__ obase, int* __restrict__ HK,
+    int* __restrict__ HV, int* __restrict__ HM, int* __restrict__ cdeg,
+    unsigned* __restrict__ okey, unsigned long long* __restrict__ oval) {
     __shared__ int n_out;
     const int i = blockIdx.x;
     if (i >= count) return;
@@ -321,26 +423,6 @@ __global__ __launch_bounds__(256) void rows_global(
     int* V = HV + hoff[i];
     int* Mn = HM + hoff[i];
     if (threadIdx.x == 0) n_out = 0;
-    __shared__ int heavy[kHeavyList];
-    __shared__ int n_heavy;
-    for_member_edges<256>(
-        members, moff[p], moff[p + 1], xadj, heavy, &n_heavy,
-        [&](int e) {
-            const int q = map[adjncy[e]];
-            if (q == p) return;
-            long long h = (long long)(mix32((uint32_t)q) & (uint32_t)(cap - 1));
-            while (true) {
-                const int old = atomicCAS(&K[h], -1, q);
-                if (old == -1 || old == q) {
-                    atomicAdd(&V[h], adjwgt[e]);
-                    atomicMin(&Mn[h], e);
-                    break;
-                }
-                h = (h + 1) & (cap - 1);
-            }
-        },
-        []() { return false; });
-    __threadfence();
     __syncthreads();
     const int base = obase[i];
     for (long long h = threadIdx.x; h < cap; h += 256) {
@@ -400,10 +482,12 @@ __global__ void project_kernel(int n, const int* __restrict__ map,
 }
 
 __global__ void hub_cap_kernel(const int* __restrict__ list, int count, const int* __restrict__ fdeg,
-                               int nc, long long* __restrict__ cap, int* __restrict__ bound) {
+                               int nc, long long* __restrict__ cap, int* __restrict__ bound,
+                               int* __restrict__ fd_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const int fd = fdeg[list[i]];
+    fd_out[i] = fd;
     const long long distinct = fd < nc ? fd : nc;
     long long c = 64;
     while (c < 2 * distinct) c <<= 1;
@@ -498,14 +582,15 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
     if (novl > 0) {
-        ProfScope ps(prof, "contract_rows_global", 0.0, 0, s);
+        ProfScope ps(prof, "contract_rows_global", 0.0, novl, s);
         hcap_.reserve(novl);
         hoff_.reserve(novl);
         obound_.reserve(novl + 1);
         obase_.reserve(novl + 1);
         oend_.reserve(novl);
+        ofd_.reserve(novl);
         hipLaunchKernelGGL(hub_cap_kernel, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl2_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p);
+                           ovl2_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p, ofd_.p);
         std::vector<long long> caps(novl), offs(novl);
         std::vector<int> bounds(novl), bases(novl);
         KH_CHECK(hipMemcpyAsync(caps.data(), hcap_.p, sizeof(long long) * novl,
@@ -535,9 +620,33 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         oval_.reserve(ototal);
         okey2_.reserve(ototal);
         oval2_.reserve(ototal);
-        hipLaunchKernelGGL(rows_global, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl, g.xadj,
-                           g.adjncy, g.adjwgt, map, moff_.p, members_.p, hoff_.p, hcap_.p,
-                           obase_.p, hk_.p, hv_.p, hm_.p, cdeg_.p, okey_.p, oval_.p);
+        // chunks of the member-edge range of each row (row capacities fdeg are on the host now)
+        std::vector<int> fd(novl), cpre(novl + 1);
+        KH_CHECK(hipMemcpyAsync(fd.data(), ofd_.p, sizeof(int) * novl, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        int nchunks = 0;
+        for (int i = 0; i < novl; ++i) {
+            cpre[i] = nchunks;
+            nchunks += (fd[i] + kGChunk - 1) / kGChunk;
+        }
+        cpre[novl] = nchunks;
+        gchunk_prefix_.reserve(novl + 1);
+        KH_CHECK(hipMemcpyAsync(gchunk_prefix_.p, cpre.data(), sizeof(int) * (novl + 1),
+                                hipMemcpyHostToDevice, s));
+        mdeg_.reserve((long long)n + 1);
+        mpre_.reserve((long long)n + 1);
+        hipLaunchKernelGGL(member_degrees, dim3(grid_for(n, 256)), dim3(256), 0, s, n, members_.p,
+                           g.xadj, mdeg_.p);
+        fill_async(mdeg_.p + n, 1, 0, s);
+        scan_i32(mdeg_.p, mpre_.p, n + 1, s);
+        if (nchunks > 0)
+            hipLaunchKernelGGL(rows_global_chunks, dim3(nchunks), dim3(kGThreads), 0, s,
+                               ovl2_list_.p, novl, gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt,
+                               map, moff_.p, members_.p, mpre_.p, hoff_.p, hcap_.p, hk_.p, hv_.p,
+                               hm_.p);
+        hipLaunchKernelGGL(rows_global_extract, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl,
+                           hoff_.p, hcap_.p, obase_.p, hk_.p, hv_.p, hm_.p, cdeg_.p, okey_.p,
+                           oval_.p);
         hipLaunchKernelGGL(overflow_ends, dim3(grid_for(novl, 256)), dim3(256), 0, s,
                            ovl2_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
         size_t bytes = 0;

@@ -20,6 +20,8 @@
 //   deg <= 512          one wave per node, LDS hash table (open addressing, LDS atomics);
 //   larger              one workgroup per node, hash table in a pre-cleared global region.
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "common.hpp"
 #include "lp.hpp"
 
@@ -34,6 +36,15 @@ constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 512;
 constexpr int kHubDense = 4096;   // dense LDS counters per hub (refinement, k <= 4096)
+constexpr int kHubDenseSlots = 16384;  // LDS counters: min(16, slots / k) private copies
+constexpr int kFinThreads = 1024;  // hub_bucket_merge workgroup
+constexpr int kFinCap = 16384;     // LDS hash slots of hub_bucket_merge (128 KB)
+constexpr int kFinLds = 8192;      // buckets up to this many entries merge in LDS
+constexpr int kMaxBuckets = 2048;  // buckets per hub: min(4 x chunks, kMaxBuckets)
+constexpr int kBucketsPerChunk = 1;  // one bucket per chunk: fewer, larger merge tasks measured faster than 4
+constexpr int kSmallThreads = 512;   // small bucket merge: <= kSmallMax entries in 32 KB of LDS
+constexpr int kSmallCap = 4096;
+constexpr int kSmallMax = 2048;
 constexpr int kHubDenseThreads = 1024;
 constexpr int kBigDegree = 64;    // moved nodes above this degree mark neighbours per workgroup
 
@@ -43,6 +54,12 @@ struct Cand {
     int lab;
     int size;
 };
+static_assert(sizeof(Cand) == sizeof(int4), "bucket results are stored as int4");
+
+// Bucket of a label among P buckets of a hub (independent of the LDS slot hash).
+__device__ inline int bucket_of(int lab, int P) {
+    return (int)(mix32((uint32_t)lab ^ 0x5bd1e995u) % (uint32_t)P);
+}
 
 __device__ inline bool better(int ac, uint32_t at, int al, int bc, uint32_t bt, int bl) {
     return ac > bc || (ac == bc && (at > bt || (at == bt && al < bl)));
@@ -90,11 +107,9 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
     if (!go) return;
     const int gain = c.conn - conn_own;
     const unsigned long long key = prio_key(gain, v);
-    const int idx = atomicAdd(a.n_movers, 1);
-    a.mv_v[idx] = v;
-    a.mv_t[idx] = c.lab;
-    a.mv_gain[idx] = gain;
-    a.mv_ok[idx] = 1;
+    a.prop_t[v] = c.lab;
+    a.prop_gain[v] = gain;
+    a.prop_ok[v] = 1;
     if (M == LpMode::Refine && overloaded) {
         atomicAdd(&a.out[own], w);
         atomicMax(&a.dtop[own], key);
@@ -255,10 +270,11 @@ __device__ inline void block_reduce(int& conn_own, Cand& c) {
 }
 
 // ---------------------------------------------------------------- hubs, refinement: dense LDS
-// One workgroup per hub; one LDS counter per block (k <= kHubDense).
+// One workgroup per hub. Each wave adds into its own copy of the per-block counters (the k blocks
+// are few, so a single shared array would serialise on LDS atomics), then the copies are summed.
 __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
     LpArgs a, const int* __restrict__ list, int count, int num_labels) {
-    __shared__ int cnt[kHubDense];
+    __shared__ int cnt[kHubDenseSlots];
     const int gi = blockIdx.x;
     if (gi >= count) return;  // uniform per block
     const int v = list[gi];
@@ -266,16 +282,19 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
     const int own = a.label[v];
     const int beg = a.xadj[v];
     const int deg = a.xadj[v + 1] - beg;
-    for (int i = threadIdx.x; i < num_labels; i += kHubDenseThreads) cnt[i] = 0;
+    const int copies = min(kHubDenseThreads / 64, kHubDenseSlots / num_labels);
+    int* mine = cnt + ((threadIdx.x >> 6) % copies) * num_labels;
+    for (int i = threadIdx.x; i < copies * num_labels; i += kHubDenseThreads) cnt[i] = 0;
     __syncthreads();
     for (int e = beg + threadIdx.x; e < beg + deg; e += kHubDenseThreads)
-        atomicAdd(&cnt[a.label[a.adjncy[e]]], a.adjwgt[e]);
+        atomicAdd(&mine[a.label[a.adjncy[e]]], a.adjwgt[e]);
     __syncthreads();
     const int nw = a.vwgt[v];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
     for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
-        const int conn = cnt[b];
+        int conn = 0;
+        for (int j = 0; j < copies; ++j) conn += cnt[j * num_labels + b];
         if (conn == 0) continue;
         if (b == own) {
             conn_own = conn;
@@ -295,14 +314,13 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
 
 // ---------------------------------------------------------------- hubs: chunked LDS hash
 // One workgroup per chunk of kHubChunk edges of a hub. The chunk's labels are aggregated in an
-// LDS hash table; a hub of one chunk decides right away, otherwise the partial sums are added
-// into the hub's global table (one atomic pair per distinct label of the chunk) and
-// hub_finalize decides.
+// LDS hash table; a hub of one chunk decides right away, otherwise the chunk's (label, partial
+// sum) entries are written out grouped by bucket(label), one bucket per later workgroup.
 template <LpMode M>
 __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const long long* __restrict__ tab_off, const long long* __restrict__ tab_cap,
-    int* __restrict__ gkey, int* __restrict__ gval) {
+    int2* __restrict__ partials, const long long* __restrict__ offm_base, int* __restrict__ offm,
+    const int* __restrict__ ct_base, int* __restrict__ cntT) {
     __shared__ int K[kHubCap];
     __shared__ int V[kHubCap];
     __shared__ int hub;
@@ -345,21 +363,56 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     }
     __syncthreads();
     if (nchunks > 1) {
-        int* GK = gkey + tab_off[gi];
-        int* GV = gval + tab_off[gi];
-        const long long cap = tab_cap[gi];
+        // The chunk's (label, partial sum) entries, grouped by bucket(label): P buckets per hub,
+        // bucket b of chunk c at [offm[c*(P+1)+b], offm[c*(P+1)+b+1]) of the chunk's region.
+        __shared__ int bcur[kMaxBuckets + 1];
+        __shared__ int wsum[kHubThreads / 64];
+        const int P = min(kBucketsPerChunk * nchunks, kMaxBuckets);
+        for (int i = threadIdx.x; i <= P; i += kHubThreads) bcur[i] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < kHubCap; i += kHubThreads)
+            if (K[i] >= 0) atomicAdd(&bcur[bucket_of(K[i], P)], 1);
+        __syncthreads();
+        // exclusive scan of bcur[0..P): each thread owns a run of buckets
+        const int per = (P + kHubThreads - 1) / kHubThreads;
+        const int b0 = threadIdx.x * per, b1 = min(P, b0 + per);
+        int run = 0;
+        for (int b = b0; b < b1; ++b) run += bcur[b];
+        int incl = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if ((threadIdx.x & 63) >= o) incl += y;
+        }
+        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        int wbase = 0;
+        for (int w = 0; w < (threadIdx.x >> 6); ++w) wbase += wsum[w];
+        int acc = wbase + incl - run;
+        for (int b = b0; b < b1; ++b) {
+            const int cnt = bcur[b];
+            bcur[b] = acc;
+            acc += cnt;
+        }
+        __syncthreads();
+        int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
+        int* cnt_t = cntT + ct_base[gi] + c_idx;  // bucket-major: [b * nchunks + c_idx]
+        for (int b = threadIdx.x; b < P; b += kHubThreads) {
+            offs[b] = bcur[b];
+            if (b + 1 < P) cnt_t[(long long)b * nchunks] = bcur[b + 1] - bcur[b];
+        }
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int w = 0; w < kHubThreads / 64; ++w) tot += wsum[w];
+            offs[P] = tot;
+            cnt_t[(long long)(P - 1) * nchunks] = tot - bcur[P - 1];
+        }
+        __syncthreads();
+        int2* out = partials + (long long)chunk * kHubChunk;
         for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
             const int lab = K[i];
             if (lab < 0) continue;
-            long long h = (long long)(mix32((uint32_t)lab) & (uint32_t)(cap - 1));
-            while (true) {
-                const int old = atomicCAS(&GK[h], -1, lab);
-                if (old == -1 || old == lab) {
-                    atomicAdd(&GV[h], V[i]);
-                    break;
-                }
-                h = (h + 1) & (cap - 1);
-            }
+            out[atomicAdd(&bcur[bucket_of(lab, P)], 1)] = make_int2(lab, V[i]);
         }
         return;
     }
@@ -387,110 +440,266 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     }
 }
 
-// One workgroup per multi-chunk hub: reads (and clears) the hub's global table and decides.
+// Moves the bucket-grouped entries of every chunk of a multi-chunk hub to their bucket-major
+// position (posT = exclusive scan of the bucket-major counts), so that every bucket of a hub is
+// one contiguous range.
+__global__ __launch_bounds__(256) void hub_scatter(
+    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    const long long* __restrict__ offm_base, const int* __restrict__ offm,
+    const int* __restrict__ ct_base, const int* __restrict__ posT,
+    const int2* __restrict__ partials, int2* __restrict__ bucketed) {
+    __shared__ int hub;
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = count;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+        }
+        hub = lo;
+    }
+    __syncthreads();
+    const int gi = hub;
+    const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    if (nch <= 1) return;
+    const int c_idx = chunk - chunk_prefix[gi];
+    const int P = min(kBucketsPerChunk * nch, kMaxBuckets);
+    const int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
+    const int* pos = posT + ct_base[gi] + c_idx;
+    const int2* src = partials + (long long)chunk * kHubChunk;
+    const int total = offs[P];
+    for (int i = threadIdx.x; i < total; i += 256) {
+        const int2 kv = src[i];
+        const int b = bucket_of(kv.x, P);
+        bucketed[pos[(long long)b * nch] + (i - offs[b])] = kv;
+    }
+}
+
+// Merge of one (multi-chunk hub, bucket) task: the bucket's entries of all chunks of the hub are
+// summed in an LDS hash (or, past kFinLds entries, in the hub's global table, filtering by
+// bucket); the bucket's best candidate and, for the bucket of the own label, conn(own) are left
+// in bres / bown. SMALL: 256 threads and 16 KB of LDS for buckets of <= kSmallMax entries;
+// larger buckets are queued for the large variant (1024 threads, 128 KB of LDS), which
+// grid-strides over that queue.
+template <LpMode M, bool SMALL>
+__global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
+    LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
+    int n_tasks, const int* __restrict__ task_prefix, const int* __restrict__ chunk_prefix,
+    const int* __restrict__ ct_base, const int* __restrict__ posT,
+    const int2* __restrict__ bucketed, const long long* __restrict__ tab_off,
+    int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
+    int* __restrict__ bown, uint8_t* __restrict__ fb, int* __restrict__ big_tasks,
+    int* __restrict__ n_big_tasks) {
+    constexpr int THREADS = SMALL ? kSmallThreads : kFinThreads;
+    constexpr int CAP = SMALL ? kSmallCap : kFinCap;
+    __shared__ int K[CAP];
+    __shared__ int V[CAP];
+    __shared__ int sh_mi;
+    const int n_iter = SMALL ? n_tasks : *n_big_tasks;
+    for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
+        const int task = SMALL ? it : big_tasks[it];
+        if (threadIdx.x == 0) {
+            int lo = 0, hi = n_multi;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (task_prefix[mid] <= task) lo = mid; else hi = mid;
+            }
+            sh_mi = lo;
+        }
+        __syncthreads();
+        const int mi = sh_mi;
+        const int gi = multi[mi];
+        const int v = list[gi];
+        if (M == LpMode::Refine && a.active[v] == 0) {  // uniform per block
+            __syncthreads();
+            continue;
+        }
+        const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
+        const int P = task_prefix[mi + 1] - task_prefix[mi];
+        const int b = task - task_prefix[mi];
+        const int r0 = posT[ct_base[gi] + b * nch];
+        const int r1 = posT[ct_base[gi] + (b + 1) * nch];
+        const int total = r1 - r0;
+        if (SMALL && total > kSmallMax) {
+            if (threadIdx.x == 0) big_tasks[atomicAdd(n_big_tasks, 1)] = task;
+            __syncthreads();
+            continue;
+        }
+        const bool in_lds = SMALL || total <= kFinLds;
+        const int deg = a.xadj[v + 1] - a.xadj[v];
+        long long cap = CAP;
+        int* TK = K;
+        int* TV = V;
+        if (in_lds) {
+            cap = 64;
+            while (cap < 2 * total) cap <<= 1;
+            for (int i = threadIdx.x; i < cap; i += THREADS) {
+                K[i] = -1;
+                V[i] = 0;
+            }
+        } else {
+            cap = 64;
+            while (cap < 2LL * deg) cap <<= 1;
+            TK = gkey + tab_off[gi];
+            TV = gval + tab_off[gi];
+            if (threadIdx.x == 0) fb[mi] = 1;
+        }
+        __syncthreads();
+        for (int i = r0 + threadIdx.x; i < r1; i += THREADS) {
+            const int2 kv = bucketed[i];
+            long long h = (long long)(mix32((uint32_t)kv.x) & (uint32_t)(cap - 1));
+            while (true) {
+                const int old = atomicCAS(&TK[h], -1, kv.x);
+                if (old == -1 || old == kv.x) {
+                    atomicAdd(&TV[h], kv.y);
+                    break;
+                }
+                h = (h + 1) & (cap - 1);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        const int own = a.label[v];
+        const int nw = a.vwgt[v];
+        int conn_own = 0;
+        Cand c{-1, 0u, 0x7fffffff, 0};
+        for (long long i = threadIdx.x; i < cap; i += THREADS) {
+            const int lab = TK[i];
+            if (lab < 0) continue;
+            if (!in_lds && bucket_of(lab, P) != b) continue;
+            const int conn = TV[i];
+            if (lab == own) {
+                conn_own = conn;
+                continue;
+            }
+            const int sz = a.size[lab];
+            if ((long long)sz + nw >= (long long)a.bound) continue;
+            const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+            if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+        }
+        block_reduce<THREADS>(conn_own, c);
+        if (threadIdx.x == 0) {
+            bres[task] = c;
+            bown[task] = conn_own;
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per multi-chunk hub: the best of its bucket winners decides; a global table used
+// by an oversized bucket is cleared here.
 template <LpMode M>
-__global__ __launch_bounds__(256) void hub_finalize(LpArgs a, const int* __restrict__ list,
-                                                    const int* __restrict__ multi, int n_multi,
-                                                    const long long* __restrict__ tab_off,
-                                                    const long long* __restrict__ tab_cap,
-                                                    int* __restrict__ gkey,
-                                                    int* __restrict__ gval) {
+__global__ __launch_bounds__(256) void hub_decide(
+    LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
+    const int* __restrict__ task_prefix, const long long* __restrict__ tab_off,
+    int* __restrict__ gkey, int* __restrict__ gval, const Cand* __restrict__ bres,
+    const int* __restrict__ bown, uint8_t* __restrict__ fb) {
     const int mi = blockIdx.x;
     if (mi >= n_multi) return;
     const int gi = multi[mi];
     const int v = list[gi];
     if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
-    int* GK = gkey + tab_off[gi];
-    int* GV = gval + tab_off[gi];
-    const long long cap = tab_cap[gi];
-    const int own = a.label[v];
-    const int nw = a.vwgt[v];
+    const int t0 = task_prefix[mi], t1 = task_prefix[mi + 1];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (long long i = threadIdx.x; i < cap; i += 256) {
-        const int lab = GK[i];
-        if (lab < 0) continue;
-        const int conn = GV[i];
-        GK[i] = -1;  // leave the region clean for the next use
-        GV[i] = 0;
-        if (lab == own) {
-            conn_own = conn;
-            continue;
-        }
-        const int sz = a.size[lab];
-        if ((long long)sz + nw >= (long long)a.bound) continue;
-        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+        conn_own = max(conn_own, bown[t]);
+        const Cand o = bres[t];
+        if (better(o.conn, o.tie, o.lab, c.conn, c.tie, c.lab)) c = o;
     }
+    const int deg = a.xadj[v + 1] - a.xadj[v];
     block_reduce<256>(conn_own, c);
     if (threadIdx.x == 0) {
-        decide<M>(a, v, own, nw, conn_own, c);
-        if (M == LpMode::Refine)
-            atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
+        decide<M>(a, v, a.label[v], a.vwgt[v], conn_own, c);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+    }
+    if (fb[mi]) {  // uniform per block
+        long long cap = 64;
+        while (cap < 2LL * deg) cap <<= 1;
+        int* TK = gkey + tab_off[gi];
+        int* TV = gval + tab_off[gi];
+        for (long long i = threadIdx.x; i < cap; i += 256) {
+            TK[i] = -1;
+            TV[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) fb[mi] = 0;
     }
 }
 
 // ---------------------------------------------------------------- filter / admit / apply
-__global__ void filter_departures(LpArgs a, int max_movers) {
+// The filter / admit / apply kernels run over the node list of the sub-round; nodes without a
+// proposal (prop_t < 0) return at once.
+__global__ void filter_departures(LpArgs a, const int* __restrict__ list, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= max_movers || i >= *a.n_movers) return;
-    const int v = a.mv_v[i];
+    if (i >= count) return;
+    const int v = list[i];
+    const int t = a.prop_t[v];
+    if (t < 0) return;
     const int own = a.label[v];
     const int so = a.size[own];
     if (so <= a.bound) return;  // not a departure from an overloaded block
     const int w = a.vwgt[v];
-    const unsigned long long key = prio_key(a.mv_gain[i], v);
+    const unsigned long long key = prio_key(a.prop_gain[v], v);
     const bool pass = (long long)so - (long long)a.out[own] >= (long long)a.bound ||
                       a.dtop[own] == key;
     if (pass) {
-        const int t = a.mv_t[i];
         atomicAdd(&a.inc[t], w);
         atomicMax(&a.top[t], key);
     } else {
-        a.mv_ok[i] = 0;
+        a.prop_ok[v] = 0;
     }
 }
 
-__global__ void admit_moves(LpArgs a, int max_movers) {
+__global__ void admit_moves(LpArgs a, const int* __restrict__ list, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= max_movers || i >= *a.n_movers) return;
-    if (!a.mv_ok[i]) return;
-    const int v = a.mv_v[i];
-    const int t = a.mv_t[i];
+    if (i >= count) return;
+    const int v = list[i];
+    const int t = a.prop_t[v];
+    if (t < 0 || !a.prop_ok[v]) return;
     const int w = a.vwgt[v];
     const long long st = a.size[t];
     const bool ok = st + (long long)a.inc[t] < (long long)a.bound ||
-                    (a.top[t] == prio_key(a.mv_gain[i], v) && st + w < (long long)a.bound);
-    if (!ok) a.mv_ok[i] = 0;
+                    (a.top[t] == prio_key(a.prop_gain[v], v) && st + w < (long long)a.bound);
+    if (!ok) a.prop_ok[v] = 0;
 }
 
 template <LpMode M>
-__global__ void apply_moves(LpArgs a, int max_movers) {
+__global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restrict__ list,
+                                                   int count) {
+    __shared__ int block_moves;
+    if (threadIdx.x == 0) block_moves = 0;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= max_movers || i >= *a.n_movers) return;
-    const int v = a.mv_v[i];
-    const int t = a.mv_t[i];
-    const int own = a.label[v];
-    a.inc[t] = 0;
-    a.top[t] = 0ull;
-    if (M == LpMode::Refine) {
-        a.out[own] = 0;
-        a.dtop[own] = 0ull;
-    }
-    if (!a.mv_ok[i]) return;
-    const int w = a.vwgt[v];
-    atomicSub(&a.size[own], w);
-    atomicAdd(&a.size[t], w);
-    a.label[v] = t;
-    atomicAdd(a.moves, 1);
-    if (M == LpMode::Refine) {
-        const int beg = a.xadj[v], end = a.xadj[v + 1];
-        if (end - beg > kBigDegree) {
-            a.big[atomicAdd(a.n_big, 1)] = v;
-        } else {
-            for (int e = beg; e < end; ++e) a.next_active[a.adjncy[e]] = 1;
+    const int v = i < count ? list[i] : 0;
+    const int t = i < count ? a.prop_t[v] : -1;
+    if (t >= 0) {
+        const int own = a.label[v];
+        a.inc[t] = 0;
+        a.top[t] = 0ull;
+        if (M == LpMode::Refine) {
+            a.out[own] = 0;
+            a.dtop[own] = 0ull;
+        }
+        a.prop_t[v] = -1;
+        if (a.prop_ok[v]) {
+            const int w = a.vwgt[v];
+            atomicSub(&a.size[own], w);
+            atomicAdd(&a.size[t], w);
+            a.label[v] = t;
+            atomicAdd(&block_moves, 1);
+            if (M == LpMode::Refine) {
+                const int beg = a.xadj[v], end = a.xadj[v + 1];
+                if (end - beg > kBigDegree) {
+                    a.big[atomicAdd(a.n_big, 1)] = v;
+                } else {
+                    for (int e = beg; e < end; ++e) a.next_active[a.adjncy[e]] = 1;
+                }
+            }
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
 }
 
 // Frontier marking for moved high-degree nodes: grid-stride over the big movers, one workgroup
@@ -649,11 +858,9 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     cursor_.reserve(L);
     offsets_.reserve(L);
     edges_.reserve(L);
-    mv_v_.reserve(g.n);
-    mv_t_.reserve(g.n);
-    mv_gain_.reserve(g.n);
-    mv_ok_.reserve(g.n);
-    n_movers_.reserve(kMaxSub * kMaxRounds);
+    prop_t_.reserve(g.n);
+    prop_gain_.reserve(g.n);
+    prop_ok_.reserve(g.n);
     moves_.reserve(kMaxRounds);
     inc_.reserve(num_labels);
     out_.reserve(num_labels);
@@ -667,6 +874,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     fill_async(counts_.p, L, 0, s);
     fill_async(cursor_.p, L, 0, s);
     fill_async(edges_.p, L, 0ull, s);
+    fill_async(prop_t_.p, g.n, -1, s);
     fill_async(inc_.p, num_labels, 0, s);
     fill_async(out_.p, num_labels, 0, s);
     fill_async(top_.p, num_labels, 0ull, s);
@@ -712,39 +920,88 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     std::vector<int> deg(hubs);
     KH_CHECK(hipMemcpyAsync(deg.data(), hub_deg_.p, sizeof(int) * hubs, hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
-    std::vector<int> prefix, multi;
-    std::vector<long long> toff(hubs, 0), tcap(hubs, 0);
+    std::vector<int> prefix, multi, tasks;
+    std::vector<long long> toff(hubs, 0), tcap(hubs, 0), obase(hubs, 0);
+    std::vector<int> ctb(hubs, 0);
+    long long offm_total = 0, ct_total = 0, ct_sr_start = 0;
+    int max_tasks = 0, max_multi = 0;
     for (int sr = 0; sr < S; ++sr) {
         chunk_base_[sr] = (int)prefix.size();
         multi_base_[sr] = (int)multi.size();
-        int c = 0;
+        task_base_[sr] = (int)tasks.size();
+        int c = 0, t = 0;
+        (void)0;
         for (int i = 0; i < hub_n_[sr]; ++i) {
             const int d = deg[hub_base_[sr] + i];
             const int nch = (d + kHubChunk - 1) / kHubChunk;
             prefix.push_back(c);
             c += nch;
             if (nch > 1) {
+                const int P = std::min(kBucketsPerChunk * nch, kMaxBuckets);
                 long long cap = 64;
                 while (cap < 2LL * d) cap <<= 1;
                 toff[hub_base_[sr] + i] = hub_total_;
                 tcap[hub_base_[sr] + i] = cap;
                 hub_total_ += cap;
+                obase[hub_base_[sr] + i] = offm_total;
+                offm_total += (long long)nch * (P + 1);
+                ctb[hub_base_[sr] + i] = (int)ct_total;
+                ct_total += (long long)nch * P;
                 multi.push_back(i);
+                tasks.push_back(t);
+                t += P;
             }
         }
         prefix.push_back(c);
+        tasks.push_back(t);
+        ct_sr_base_[sr] = (int)ct_sr_start;
+        ct_sr_size_[sr] = (int)(ct_total - ct_sr_start);
+        ct_total += 1;  // zero sentinel after the sub-round's counts
+        ct_sr_start = ct_total;
         hub_chunks_[sr] = c;
         hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
+        hub_tasks_[sr] = t;
+        max_tasks = std::max(max_tasks, t);
+        max_multi = std::max(max_multi, hub_multi_n_[sr]);
     }
     chunk_prefix_.reserve((long long)prefix.size());
     tab_off_.reserve(hubs);
     tab_cap_.reserve(hubs);
+    offm_base_.reserve(hubs);
+    task_prefix_.reserve((long long)tasks.size());
     multi_.reserve((long long)multi.size());
+    offm_.reserve(offm_total);
+    KH_REQUIRE(ct_total < (1LL << 31), "hub bucket table exceeds 2^31 entries");
+    ct_base_.reserve(hubs);
+    cnt_t_.reserve(ct_total);
+    pos_t_.reserve(ct_total);
+    fill_async(cnt_t_.p, ct_total, 0, s);  // sentinels stay zero
+    KH_CHECK(hipMemcpyAsync(ct_base_.p, ctb.data(), sizeof(int) * hubs, hipMemcpyHostToDevice,
+                            s));
+    {
+        size_t bytes = 0, need = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            KH_CHECK(rocprim::exclusive_scan(nullptr, bytes, cnt_t_.p, pos_t_.p, 0,
+                                             (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
+            need = std::max(need, bytes);
+        }
+        scan_tmp_.reserve((long long)need + 1);
+        scan_tmp_bytes_ = need;
+    }
+    bres_.reserve(max_tasks);
+    bown_.reserve(max_tasks);
+    big_tasks_.reserve(max_tasks);
+    fb_.reserve(max_multi);
+    fill_async(fb_.p, max_multi, (uint8_t)0, s);
     KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
                             hipMemcpyHostToDevice, s));
     KH_CHECK(hipMemcpyAsync(tab_off_.p, toff.data(), sizeof(long long) * hubs,
                             hipMemcpyHostToDevice, s));
     KH_CHECK(hipMemcpyAsync(tab_cap_.p, tcap.data(), sizeof(long long) * hubs,
+                            hipMemcpyHostToDevice, s));
+    KH_CHECK(hipMemcpyAsync(offm_base_.p, obase.data(), sizeof(long long) * hubs,
+                            hipMemcpyHostToDevice, s));
+    KH_CHECK(hipMemcpyAsync(task_prefix_.p, tasks.data(), sizeof(int) * tasks.size(),
                             hipMemcpyHostToDevice, s));
     if (!multi.empty())
         KH_CHECK(hipMemcpyAsync(multi_.p, multi.data(), sizeof(int) * multi.size(),
@@ -755,6 +1012,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         fill_async(hub_key_.p, hub_total_, -1, s);
         fill_async(hub_val_.p, hub_total_, 0, s);
     }
+    int max_chunks = 0;
+    for (int sr = 0; sr < S; ++sr) max_chunks = std::max(max_chunks, hub_chunks_[sr]);
+    partials_.reserve((long long)max_chunks * kHubChunk);
+    bucketed_.reserve((long long)max_chunks * kHubChunk);
     KH_CHECK(hipStreamSynchronize(s));  // the host vectors above must outlive the copies
     KH_CHECK(hipGetLastError());
 }
@@ -767,8 +1028,9 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     const int S = subrounds_;
     const bool cl = mode == LpMode::Cluster;
     KH_REQUIRE(cl || !(dense_wave_ || dense_hub_) || num_labels_ > 0, "prepare() not called");
-    fill_async(n_movers_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
+    n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
+    fill_async(n_big_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
     trav_.reserve(64);
     fill_async(trav_.p, 64, 0ull, s);
@@ -788,10 +1050,9 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.top = top_.p;
     a.out = out_.p;
     a.dtop = dtop_.p;
-    a.mv_v = mv_v_.p;
-    a.mv_t = mv_t_.p;
-    a.mv_gain = mv_gain_.p;
-    a.mv_ok = mv_ok_.p;
+    a.prop_t = prop_t_.p;
+    a.prop_gain = prop_gain_.p;
+    a.prop_ok = prop_ok_.p;
     a.trav = trav_.p;
     a.big = big_.p;
     uint8_t* act = active_.p;
@@ -804,7 +1065,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         a.active = cl ? nullptr : act;
         a.next_active = cl ? nullptr : nxt;
         for (int sr = 0; sr < S; ++sr) {
-            a.n_movers = n_movers_.p + (r * S + sr);
             a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
             for (int b = 0; b < kNumBins; ++b) {
@@ -846,13 +1106,33 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         }
                         const int* cp = chunk_prefix_.p + chunk_base_[sr];
                         const long long* to = tab_off_.p + hub_base_[sr];
-                        const long long* tc = tab_cap_.p + hub_base_[sr];
-                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, to, tc, hub_key_.p, hub_val_.p);
-                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, to, tc, hub_key_.p, hub_val_.p);
+                        const long long* ob = offm_base_.p + hub_base_[sr];
+                        const int* ctb = ct_base_.p + hub_base_[sr];
+                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         if (hub_multi_n_[sr] > 0) {
+                            ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);
                             const int* ml = multi_.p + multi_base_[sr];
-                            if (cl) hipLaunchKernelGGL((hub_finalize<LpMode::Cluster>), dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, ml, hub_multi_n_[sr], to, tc, hub_key_.p, hub_val_.p);
-                            else hipLaunchKernelGGL((hub_finalize<LpMode::Refine>), dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, ml, hub_multi_n_[sr], to, tc, hub_key_.p, hub_val_.p);
+                            const int* tp = task_prefix_.p + task_base_[sr];
+                            const int nm = hub_multi_n_[sr];
+                            Cand* br = reinterpret_cast<Cand*>(bres_.p);
+                            const int nt = hub_tasks_[sr];
+                            // bucket-major positions, relative to this sub-round's block of counts
+                            size_t sb = scan_tmp_bytes_;
+                            KH_CHECK(rocprim::exclusive_scan(scan_tmp_.p, sb, cnt_t_.p + ct_sr_base_[sr], pos_t_.p + ct_sr_base_[sr], 0, (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
+                            const int* pt = pos_t_.p;
+                            hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, lst, cnt, cp, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
+                            int* bt = big_tasks_.p;
+                            int* nbt = n_big_tasks_.p + (r * S + sr);
+                            if (cl) {
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
+                            } else {
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
+                            }
                         }
                         break;
                     }
@@ -860,13 +1140,14 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
             }
             if (sub_total == 0) continue;
             const int gm = grid_for(sub_total, 256);
+            const int* sub_list = lists_.p + host_offsets[sr * kNumBins];  // bins are contiguous
             ProfScope ps(prof, base + "_admit_apply", 0.0, 0, s);
-            if (!cl) hipLaunchKernelGGL(filter_departures, dim3(gm), dim3(256), 0, s, a, sub_total);
-            hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_total);
+            if (!cl) hipLaunchKernelGGL(filter_departures, dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
+            hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
             if (cl) {
-                hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_total);
+                hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
             } else {
-                hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_total);
+                hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
                 hipLaunchKernelGGL(mark_big_neighbours, dim3(std::min(gm, 1024)), dim3(256), 0, s, a);
             }
         }

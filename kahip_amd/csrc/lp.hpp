@@ -25,11 +25,9 @@ struct LpArgs {
     unsigned long long* top;   // per label: top proposal priority into it
     int* out;                  // per label: summed weight of departures (refinement)
     unsigned long long* dtop;  // per label: top departure priority (refinement)
-    int* mv_v;                 // mover records
-    int* mv_t;
-    int* mv_gain;
-    int* mv_ok;
-    int* n_movers;
+    int* prop_t;               // per node: proposed label, -1 outside propose..apply
+    int* prop_gain;
+    uint8_t* prop_ok;
     int* moves;
     int* big;                  // refinement: moved nodes of degree > kBigDegree
     int* n_big;
@@ -72,11 +70,24 @@ private:
     int hub_base_[kMaxSub] = {};    // offset into tab_off_/tab_cap_ (list order)
     int chunk_base_[kMaxSub] = {};  // offset into chunk_prefix_
     int multi_base_[kMaxSub] = {};  // offset into multi_
+    int task_base_[kMaxSub] = {};   // offset into task_prefix_
+    int hub_tasks_[kMaxSub] = {};   // (hub, bucket) tasks per sub-round
+    int ct_sr_base_[kMaxSub] = {};  // bucket-major count block of each sub-round
+    int ct_sr_size_[kMaxSub] = {};
+    size_t scan_tmp_bytes_ = 0;
     DevBuf<int> hub_deg_, chunk_prefix_, multi_, big_, n_big_;
     DevBuf<long long> tab_off_, tab_cap_;
+    DevBuf<int2> partials_, bucketed_;
+    DevBuf<int> ct_base_, cnt_t_, pos_t_;
+    DevBuf<char> scan_tmp_;
+    DevBuf<long long> offm_base_;
+    DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_;
+    DevBuf<int4> bres_;
+    DevBuf<uint8_t> fb_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
     DevBuf<unsigned long long> edges_;
-    DevBuf<int> mv_v_, mv_t_, mv_gain_, mv_ok_, n_movers_, moves_;
+    DevBuf<int> prop_t_, prop_gain_, moves_;
+    DevBuf<uint8_t> prop_ok_;
     DevBuf<int> inc_, out_;
     DevBuf<unsigned long long> trav_;
     DevBuf<unsigned long long> top_, dtop_;
