@@ -29,7 +29,24 @@ typedef struct {
     int ip_threads;      /* CPU threads of the initial partitioner; 0 = automatic */
     int bins[6];         /* upper degree bounds of the LP kernel bins; bins[4] <= 512 */
     int profile;         /* record per-launch timings (kahip_profile_json) */
+    int rating;          /* edge rating of the LP clustering: KAHIP_RATING_* */
+    int rating_levels;   /* the rating applies to coarsening levels < rating_levels */
+    const float* mlp_weights; /* KAHIP_MLP_PARAMS floats (host), for KAHIP_RATING_LEARNED */
 } kahip_params;
+
+/* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);
+ * EXPANSION_STAR2 is w(e)^2 / (c(u) c(v)) (lib/partition/coarsening/edge_rating/
+ * edge_ratings.cpp:171-183) quantised as clamp(round(1024 w^2 / (c(u) c(v))), 1, 4096);
+ * LEARNED is the MLP 8 -> 64 -> 64 -> 1 on the per-edge features of kahip_edge_features,
+ * quantised as clamp(round(128 logit) + 2048, 1, 4096). Scores are shifted right by s >= 0,
+ * the smallest shift with max_degree * (4096 >> s) < 2^31, and kept >= 1. */
+#define KAHIP_RATING_NONE 0
+#define KAHIP_RATING_EXPANSION_STAR2 1
+#define KAHIP_RATING_LEARNED 2
+#define KAHIP_MLP_FEATURES 8
+#define KAHIP_MLP_HIDDEN 64
+/* W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3, row-major FP32 */
+#define KAHIP_MLP_PARAMS 4801
 
 #define KAHIP_MAX_LEVELS 64
 
@@ -44,6 +61,7 @@ typedef struct {
     double ms_coarsen, ms_initial, ms_refine, ms_cut;
     long long edges_cluster, edges_contract, edges_refine, edges_cut;
     long long max_block_weight;
+    double ms_rating; /* rating time, part of ms_coarsen */
 } kahip_stats;
 
 void kahip_default_params(kahip_params* p);
@@ -96,6 +114,36 @@ int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_pa
 int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
                             const int* adjwgt, int k, long long block_bound, int repetitions,
                             unsigned long long seed, int* part);
+
+/* Learned edge rating (no reference counterpart; the reference rates edges in
+ * lib/partition/coarsening/edge_rating/edge_ratings.cpp:171-183 and does not use the rating on
+ * the LP path). features: xadj[n] x 8 floats per directed edge (u, v):
+ *   0 log2(1 + min(deg u, deg v))  1 log2(1 + max deg)  2 log2(1 + w(e))
+ *   3 log2(1 + min(c(u), c(v)))    4 log2(1 + max c)    5 popcount(sig(u) & sig(v)) / 8
+ *   6 number of 3 one-round LP clusterings (cluster bound `bound`, seeds hash3(seed, 0xA6, i))
+ *     that put u and v together
+ *   7 log2(1 + w(e)^2 / max(c(u) c(v), 1))
+ * with sig(x) = OR over neighbours y of 1 << (mix32(y) & 63). */
+int kahip_edge_features(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                        const int* adjwgt, int bound, unsigned seed, float* features);
+int kahip_graph_edge_features(kahip_graph_t g, int bound, unsigned seed, float* features);
+/* MLP logits (BF16 operands on MFMA, FP32 accumulate) of count feature rows. */
+int kahip_mlp_forward(long long count, const float* features, const float* params,
+                      float* logits);
+/* Integer scores of a rating (KAHIP_RATING_*) for every directed edge. */
+int kahip_edge_scores(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                      const int* adjwgt, int rating, const float* params, int bound,
+                      unsigned seed, int* scores);
+/* Mean binary cross-entropy (labels in {0, 1}) of the MLP and its gradient, computed on the
+ * GPU in fixed order. The forward rounds inputs, weights and layer-1 activations to BF16 like
+ * the MFMA path; the backward treats those roundings as the identity. */
+int kahip_mlp_gradient(int count, const float* features, const float* labels,
+                       const float* params, float* grads, double* loss);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` passes of consecutive mini-batches of the
+ * rows in the given order; params is updated in place, loss_per_epoch (may be NULL) receives
+ * each pass's mean of batch losses. Deterministic. */
+int kahip_mlp_train(long long count, const float* features, const float* labels, int batch,
+                    int epochs, float lr, float* params, double* loss_per_epoch);
 
 #ifdef __cplusplus
 }

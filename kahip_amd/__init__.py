@@ -27,9 +27,13 @@ LIB_PATH = os.path.join(HERE, "lib", "libkahip.so")
 
 FAST, ECO, STRONG, FASTSOCIAL, ECOSOCIAL, STRONGSOCIAL = range(6)
 MAX_LEVELS = 64
+RATING_NONE, RATING_EXPANSION_STAR2, RATING_LEARNED = 0, 1, 2
+MLP_FEATURES, MLP_HIDDEN, MLP_PARAMS = 8, 64, 4801
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
 class KahipError(RuntimeError):
@@ -41,7 +45,8 @@ class Params(C.Structure):
                 ("cluster_rounds", C.c_int), ("refine_rounds", C.c_int), ("subrounds", C.c_int),
                 ("cluster_factor", C.c_int), ("stop_factor", C.c_int),
                 ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
-                ("profile", C.c_int)]
+                ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
+                ("mlp_weights", C.POINTER(C.c_float))]
 
 
 class Stats(C.Structure):
@@ -53,7 +58,7 @@ class Stats(C.Structure):
                 ("ms_refine", C.c_double), ("ms_cut", C.c_double),
                 ("edges_cluster", C.c_longlong), ("edges_contract", C.c_longlong),
                 ("edges_refine", C.c_longlong), ("edges_cut", C.c_longlong),
-                ("max_block_weight", C.c_longlong)]
+                ("max_block_weight", C.c_longlong), ("ms_rating", C.c_double)]
 
     def as_dict(self):
         L = self.levels
@@ -66,7 +71,7 @@ class Stats(C.Structure):
             "ms_refine": self.ms_refine, "ms_cut": self.ms_cut,
             "edges_cluster": int(self.edges_cluster), "edges_contract": int(self.edges_contract),
             "edges_refine": int(self.edges_refine), "edges_cut": int(self.edges_cut),
-            "max_block_weight": int(self.max_block_weight),
+            "max_block_weight": int(self.max_block_weight), "ms_rating": self.ms_rating,
         }
 
 
@@ -76,7 +81,9 @@ EXPORTS = [
     "kaffpa_gpu_ex", "kahip_graph_from_host", "kahip_graph_rmat", "kahip_graph_info",
     "kahip_graph_to_host", "kahip_graph_free", "kahip_partition", "kahip_profile_json",
     "kahip_lp_cluster", "kahip_remap", "kahip_contract", "kahip_lp_refine", "kahip_edge_cut",
-    "kahip_block_weights", "kahip_project", "kahip_initial_partition",
+    "kahip_block_weights", "kahip_project", "kahip_initial_partition", "kahip_edge_features",
+    "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
+    "kahip_mlp_train",
 ]
 
 _lib = None
@@ -132,6 +139,15 @@ def lib():
     L.kahip_project.argtypes = [C.c_int, _i32p, C.c_int, _i32p, _i32p]
     L.kahip_initial_partition.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
                                           C.c_int, C.c_ulonglong, _i32p]
+    L.kahip_edge_features.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_uint, _f32p]
+    L.kahip_graph_edge_features.argtypes = [G, C.c_int, C.c_uint, _f32p]
+    L.kahip_mlp_forward.argtypes = [C.c_longlong, _f32p, _f32p, _f32p]
+    L.kahip_edge_scores.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int,
+                                    C.POINTER(C.c_float), C.c_int, C.c_uint, _i32p]
+    L.kahip_mlp_gradient.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p,
+                                     C.POINTER(C.c_double)]
+    L.kahip_mlp_train.argtypes = [C.c_longlong, _f32p, _f32p, C.c_int, C.c_int, C.c_float,
+                                  _f32p, _f64p]
     _lib = L
     return L
 
@@ -153,12 +169,23 @@ def _i32(a):
 
 
 def default_params(**kw) -> Params:
+    """kahip_default_params() with overrides; ``mlp_weights`` takes an array of MLP_PARAMS
+    floats (kept alive by the returned object)."""
     p = Params()
     lib().kahip_default_params(C.byref(p))
     for k, v in kw.items():
         if k == "bins":
             for i, b in enumerate(v):
                 p.bins[i] = b
+        elif k == "mlp_weights":
+            if v is None:
+                p.mlp_weights = None
+                continue
+            w = np.ascontiguousarray(v, dtype=np.float32)
+            if w.size != MLP_PARAMS:
+                raise KahipError(f"mlp_weights must hold {MLP_PARAMS} floats")
+            p._mlp_keep = w
+            p.mlp_weights = w.ctypes.data_as(C.POINTER(C.c_float))
         else:
             setattr(p, k, v)
     return p
@@ -353,3 +380,78 @@ def initial_partition(xadj, adjncy, vwgt, adjwgt, k, block_bound, repetitions=-1
                                          _p(vwgt), _p(adjwgt), k, int(block_bound), repetitions,
                                          seed, out))
     return out[:n]
+
+
+# ----------------------------------------------------------------------------- edge ratings
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def edge_features(xadj, adjncy, vwgt, adjwgt, bound, seed):
+    """Per-directed-edge features (xadj[n] x 8 float32) of the learned rating."""
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    m2 = int(xadj[-1])
+    out = np.zeros((max(m2, 1), MLP_FEATURES), np.float32)
+    _check(lib().kahip_edge_features(n, xadj, adjncy if m2 else np.zeros(1, np.int32), _p(vwgt),
+                                     _p(adjwgt), int(bound), seed & 0xFFFFFFFF, out))
+    return out[:m2]
+
+
+def graph_edge_features(graph, bound, seed):
+    m2 = graph.info()["m2"]
+    out = np.zeros((max(m2, 1), MLP_FEATURES), np.float32)
+    _check(lib().kahip_graph_edge_features(graph._h, int(bound), seed & 0xFFFFFFFF, out))
+    return out[:m2]
+
+
+def mlp_forward(features, params):
+    """MLP logits of feature rows (BF16 on MFMA, FP32 accumulate)."""
+    f = _f32(features).reshape(-1, MLP_FEATURES)
+    m = f.shape[0]
+    out = np.zeros(max(m, 1), np.float32)
+    _check(lib().kahip_mlp_forward(m, f if m else np.zeros((1, 8), np.float32), _f32(params),
+                                   out))
+    return out[:m]
+
+
+def edge_scores(xadj, adjncy, vwgt, adjwgt, rating, params=None, bound=0, seed=0):
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    m2 = int(xadj[-1])
+    out = np.zeros(max(m2, 1), np.int32)
+    w = None if params is None else _f32(params)
+    _check(lib().kahip_edge_scores(n, xadj, adjncy if m2 else np.zeros(1, np.int32), _p(vwgt),
+                                   _p(adjwgt), rating,
+                                   None if w is None else w.ctypes.data_as(C.POINTER(C.c_float)),
+                                   int(bound), seed & 0xFFFFFFFF, out))
+    return out[:m2]
+
+
+def mlp_gradient(features, labels, params):
+    """(mean BCE loss, mean gradient) of the MLP on the GPU."""
+    f = _f32(features).reshape(-1, MLP_FEATURES)
+    y = _f32(labels)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_double(0)
+    _check(lib().kahip_mlp_gradient(f.shape[0], f, y, _f32(params), g, C.byref(loss)))
+    return loss.value, g
+
+
+def mlp_train(features, labels, params, batch=4096, epochs=1, lr=1e-3):
+    """Adam on the GPU over consecutive mini-batches; returns (params, loss per epoch)."""
+    f = _f32(features).reshape(-1, MLP_FEATURES)
+    y = _f32(labels)
+    w = _f32(params).copy()
+    losses = np.zeros(max(epochs, 1), np.float64)
+    _check(lib().kahip_mlp_train(f.shape[0], f, y, batch, epochs, lr, w, losses))
+    return w, losses[:epochs]
+
+
+def load_rating_weights(path=None):
+    """The committed trained MLP weights (float32[MLP_PARAMS])."""
+    path = path or os.path.join(HERE, "data", "rating_mlp.npy")
+    w = np.load(path, allow_pickle=False).astype(np.float32)
+    if w.size != MLP_PARAMS:
+        raise KahipError(f"{path}: expected {MLP_PARAMS} floats")
+    return w

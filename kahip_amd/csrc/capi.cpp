@@ -58,6 +58,11 @@ PartitionParams to_params(const kahip_params* q) {
     p.ip_threads = q->ip_threads;
     for (int b = 0; b < kNumBins; ++b) p.bins.hi[b] = q->bins[b];
     p.profile = q->profile != 0;
+    KH_REQUIRE(q->rating >= 0 && q->rating <= 2, "bad rating");
+    p.rating = (RatingMode)q->rating;
+    p.rating_levels = q->rating_levels;
+    p.mlp = reinterpret_cast<const MlpWeights*>(q->mlp_weights);
+    KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
         KH_REQUIRE(p.bins.hi[b] < p.bins.hi[b + 1], "bins must be strictly increasing");
@@ -117,6 +122,7 @@ void fill_stats(const PartitionStats& s, kahip_stats* o) {
     o->edges_refine = s.edges_refine;
     o->edges_cut = s.edges_cut;
     o->max_block_weight = s.max_block_weight;
+    o->ms_rating = s.ms_rating;
 }
 
 // Greedy CPU rebalancing for kaffpa_balance(perfectly_balance): while a block exceeds `bound`,
@@ -221,6 +227,9 @@ void kahip_default_params(kahip_params* p) {
     p->ip_threads = d.ip_threads;
     for (int b = 0; b < kNumBins; ++b) p->bins[b] = d.bins.hi[b];
     p->profile = 0;
+    p->rating = (int)d.rating;
+    p->rating_levels = d.rating_levels;
+    p->mlp_weights = nullptr;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }
@@ -549,6 +558,150 @@ int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int
         std::vector<int> out;
         initial_partition(g, p, out);
         std::copy(out.begin(), out.end(), part);
+    });
+}
+
+int kahip_edge_features(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                        const int* adjwgt, int bound, unsigned seed, float* features) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, vwgt, adjwgt, g, s);
+        DevBuf<float> f;
+        f.reserve(std::max<long long>(g.g.m2 * kFeatures, 1));
+        eng.rw.features(g.g, bound, seed, f.p, s, eng.lp, nullptr);
+        if (g.g.m2)
+            KH_CHECK(hipMemcpyAsync(features, f.p, sizeof(float) * kFeatures * g.g.m2,
+                                    hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_graph_edge_features(kahip_graph_t g, int bound, unsigned seed, float* features) {
+    return guarded([&] {
+        KH_REQUIRE(g, "null graph");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        const DGraph& d = g->og.g;
+        DevBuf<float> f;
+        f.reserve(std::max<long long>(d.m2 * kFeatures, 1));
+        eng.rw.features(d, bound, seed, f.p, s, eng.lp, nullptr);
+        if (d.m2)
+            KH_CHECK(hipMemcpyAsync(features, f.p, sizeof(float) * kFeatures * d.m2,
+                                    hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_mlp_forward(long long count, const float* features, const float* params,
+                      float* logits) {
+    return guarded([&] {
+        KH_REQUIRE(count >= 0, "count must be non-negative");
+        KH_REQUIRE(params, "params is NULL");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<float> f, o;
+        f.reserve(std::max<long long>(count * kFeatures, 1));
+        o.reserve(std::max<long long>(count, 1));
+        if (count)
+            KH_CHECK(hipMemcpyAsync(f.p, features, sizeof(float) * kFeatures * count,
+                                    hipMemcpyHostToDevice, s));
+        eng.rw.mlp_forward(f.p, count, *reinterpret_cast<const MlpWeights*>(params), o.p, s,
+                           nullptr);
+        if (count)
+            KH_CHECK(hipMemcpyAsync(logits, o.p, sizeof(float) * count, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_edge_scores(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                      const int* adjwgt, int rating, const float* params, int bound,
+                      unsigned seed, int* scores) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(rating == 1 || rating == 2, "rating must be EXPANSION_STAR2 or LEARNED");
+        KH_REQUIRE(rating != 2 || params, "learned rating needs params");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        OwnedGraph g;
+        upload_graph(n, xadj, adjncy, vwgt, adjwgt, g, s);
+        DevBuf<int> o;
+        o.reserve(std::max<long long>(g.g.m2, 1));
+        eng.rw.scores((RatingMode)rating, g.g, bound, seed,
+                      reinterpret_cast<const MlpWeights*>(params), o.p, s, eng.lp, nullptr);
+        if (g.g.m2)
+            KH_CHECK(hipMemcpyAsync(scores, o.p, sizeof(int) * g.g.m2, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_mlp_gradient(int count, const float* features, const float* labels,
+                       const float* params, float* grads, double* loss) {
+    return guarded([&] {
+        KH_REQUIRE(count > 0, "count must be positive");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<float> f, y, w, g, l;
+        f.reserve((long long)count * kFeatures);
+        y.reserve(count);
+        w.reserve(kMlpParams);
+        g.reserve(kMlpParams);
+        l.reserve(1);
+        KH_CHECK(hipMemcpyAsync(f.p, features, sizeof(float) * kFeatures * count,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(y.p, labels, sizeof(float) * count, hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(w.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+        eng.rw.train_step(f.p, y.p, count, w.p, nullptr, nullptr, g.p, 1, 0.f, l.p, s);
+        float hl = 0;
+        KH_CHECK(hipMemcpyAsync(grads, g.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipMemcpyAsync(&hl, l.p, sizeof(float), hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        if (loss) *loss = hl;
+    });
+}
+
+int kahip_mlp_train(long long count, const float* features, const float* labels, int batch,
+                    int epochs, float lr, float* params, double* loss_per_epoch) {
+    return guarded([&] {
+        KH_REQUIRE(count > 0 && batch > 0 && epochs >= 0, "bad count / batch / epochs");
+        KH_REQUIRE(lr > 0, "lr must be positive");
+        Engine eng;
+        hipStream_t s = eng.stream();
+        DevBuf<float> f, y, w, m, v, g, l;
+        f.reserve(count * kFeatures);
+        y.reserve(count);
+        w.reserve(kMlpParams);
+        m.reserve(kMlpParams);
+        v.reserve(kMlpParams);
+        g.reserve(kMlpParams);
+        const long long steps_per_epoch = (count + batch - 1) / batch;
+        l.reserve(std::max<long long>(steps_per_epoch, 1));
+        KH_CHECK(hipMemcpyAsync(f.p, features, sizeof(float) * kFeatures * count,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(y.p, labels, sizeof(float) * count, hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(w.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+        std::vector<float> zeros(kMlpParams, 0.f), hl(steps_per_epoch);
+        KH_CHECK(hipMemcpyAsync(m.p, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(v.p, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+        int step = 0;
+        for (int ep = 0; ep < epochs; ++ep) {
+            for (long long i = 0; i < steps_per_epoch; ++i) {
+                const long long lo = i * batch;
+                const int b = (int)std::min<long long>(batch, count - lo);
+                eng.rw.train_step(f.p + lo * kFeatures, y.p + lo, b, w.p, m.p, v.p, g.p, ++step,
+                                  lr, l.p + i, s);
+            }
+            KH_CHECK(hipMemcpyAsync(hl.data(), l.p, sizeof(float) * steps_per_epoch,
+                                    hipMemcpyDeviceToHost, s));
+            KH_CHECK(hipStreamSynchronize(s));
+            double tot = 0;
+            for (float x : hl) tot += x;
+            if (loss_per_epoch) loss_per_epoch[ep] = tot / (double)steps_per_epoch;
+        }
+        KH_CHECK(hipMemcpyAsync(params, w.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
     });
 }
 

@@ -120,6 +120,7 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
     KH_REQUIRE(p.k <= 65536, "nparts must be <= 65536");
     KH_REQUIRE(p.imbalance >= 0, "imbalance must be >= 0");
     KH_REQUIRE(g0.total_vwgt < (1LL << 31), "total node weight must be < 2^31");
+    KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs MLP weights");
     PartitionStats local;
     PartitionStats& S = st ? *st : local;
     S = PartitionStats();
@@ -138,7 +139,7 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
     std::vector<std::unique_ptr<OwnedGraph>> coarse;  // coarse[l] = level l+1
     std::vector<DevBuf<int>> maps;                     // maps[l]: level l -> level l+1
     auto level_graph = [&](int l) -> const DGraph& { return l == 0 ? g0 : coarse[l - 1]->g; };
-    DevBuf<int> label, size;
+    DevBuf<int> label, size, score;
     EventPair ev_coarsen;
     KH_CHECK(hipEventRecord(ev_coarsen.a, s));
     const unsigned num_stop = (unsigned)p.stop_factor * (unsigned)p.k;
@@ -153,8 +154,21 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
         size.reserve(fine.n);
         LpWorkspace::init_cluster_labels(fine, label.p, size.p, s);
         const uint32_t cseed = cluster_seed(p.seed, level);
-        lp.prepare(LpMode::Cluster, fine, fine.n, p.subrounds, cseed, p.bins, s);
-        lp.run(LpMode::Cluster, fine, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
+        // The clustering sees the rating in place of the edge weights; contraction keeps them.
+        DGraph rated = fine;
+        if (p.rating != RatingMode::None && level < p.rating_levels && fine.m2 > 0) {
+            EventPair ev;
+            KH_CHECK(hipEventRecord(ev.a, s));
+            score.reserve(fine.m2);
+            rw.scores(p.rating, fine, (int)B, rating_seed(p.seed, level), p.mlp, score.p, s, lp,
+                      prof);
+            KH_CHECK(hipEventRecord(ev.b, s));
+            KH_CHECK(hipEventSynchronize(ev.b));
+            S.ms_rating += ev.ms();
+            rated.adjwgt = score.p;
+        }
+        lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, cseed, p.bins, s);
+        lp.run(LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
                "lp_cluster");
         long long edges_round = 0;
         for (int i = 0; i < p.subrounds * kNumBins; ++i) edges_round += lp.host_edges[i];

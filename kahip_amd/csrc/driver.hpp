@@ -12,8 +12,11 @@
 #include "initial_partition.hpp"
 #include "lp.hpp"
 #include "metrics.hpp"
+#include "rating.hpp"
 
 namespace kahip {
+
+constexpr int kMaxLevels = 64;
 
 struct PartitionParams {
     int k = 2;
@@ -28,9 +31,12 @@ struct PartitionParams {
     int ip_threads = 0;
     BinConfig bins;
     bool profile = false;
+    // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels
+    // (None: the edge weights themselves, as in the FASTSOCIAL preset).
+    RatingMode rating = RatingMode::None;
+    int rating_levels = kMaxLevels;
+    const MlpWeights* mlp = nullptr;  // required for RatingMode::Learned
 };
-
-constexpr int kMaxLevels = 64;
 
 struct PartitionStats {
     long long cut = 0;
@@ -41,6 +47,7 @@ struct PartitionStats {
     int level_n[kMaxLevels + 1] = {};
     long long level_m2[kMaxLevels + 1] = {};
     double ms_coarsen = 0, ms_initial = 0, ms_refine = 0, ms_cut = 0;
+    double ms_rating = 0;  // part of ms_coarsen
     long long edges_cluster = 0, edges_contract = 0, edges_refine = 0, edges_cut = 0;
     long long max_block_weight = 0;
 };
@@ -51,6 +58,9 @@ inline uint32_t cluster_seed(int seed, int level) {
 }
 inline uint32_t refine_seed(int seed, int level) {
     return hash3((uint32_t)seed, 0x4Eu, (uint32_t)level);
+}
+inline uint32_t rating_seed(int seed, int level) {
+    return hash3((uint32_t)seed, 0x5Au, (uint32_t)level);
 }
 
 // app/balance_configuration.h:40-46 with the interface's percent round trip
@@ -76,6 +86,7 @@ public:
     LpWorkspace lp;
     ContractWorkspace cw;
     MetricsWorkspace mw;
+    RatingWorkspace rw;
 
 private:
     hipStream_t s_ = nullptr;

@@ -1,0 +1,525 @@
+// Edge ratings for the LP clustering on gfx950.
+//
+// Plain: expansion*^2 = w(e)^2 / (c(u) c(v)) (edge_ratings.cpp:171-183), quantised linearly:
+//   score = clamp(round(1024 * w^2 / (c(u) c(v))), 1, kScoreMax).
+// Learned: features -> MLP -> logit -> score = clamp(round(128 * logit) + 2048, 1, kScoreMax).
+// Features of a directed edge e = (u, v), symmetric in u and v, from integers only:
+//   f0 log2(1 + min deg)   f1 log2(1 + max deg)    f2 log2(1 + w(e))
+//   f3 log2(1 + min c)     f4 log2(1 + max c)      f5 popcount(sig(u) & sig(v)) / 8
+//   f6 number of 3 seeded one-round LP sweeps that put u and v in the same cluster
+//   f7 log2(1 + expansion*^2)
+// sig(x) is a 64-bit sketch of x's neighbourhood (bit mix32(y) & 63 set for every neighbour y),
+// so f5 is a capped, collision-prone common-neighbour count computed in O(1) per edge.
+//
+// MLP on MFMA (v_mfma_f32_16x16x32_bf16): a wave evaluates 16 edges at a time with the edges
+// along N. The layer-1 output tile of a lane (4 hidden units of one edge) is exactly what the
+// B operand of layer 2 needs once the K order of each 32-wide K block is permuted (the packed
+// W2 uses the same permutation), so activations never leave registers. Layer 3 (64 -> 1) is a
+// per-lane dot product plus two cross-lane adds.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rating.hpp"
+
+namespace kahip {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
+
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+__device__ inline float round_bf(float f) { return bf2f(f2bf(f)); }
+
+// ------------------------------------------------------------------ features
+__global__ void sketch_kernel(int n, const int* __restrict__ xadj, const int* __restrict__ adjncy,
+                              unsigned long long* __restrict__ sig) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    unsigned long long s = 0;
+    for (int e = xadj[u]; e < xadj[u + 1]; ++e) s |= 1ull << (mix32((uint32_t)adjncy[e]) & 63u);
+    sig[u] = s;
+}
+
+__device__ inline float lg(float x) { return __log2f(1.0f + x); }
+
+__global__ __launch_bounds__(256) void features_kernel(
+    int n, long long m2, const int* __restrict__ xadj, const int* __restrict__ adjncy,
+    const int* __restrict__ adjwgt, const int* __restrict__ vwgt,
+    const unsigned long long* __restrict__ sig, const int* __restrict__ l0,
+    const int* __restrict__ l1, const int* __restrict__ l2, float* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long e0 = t * 8;
+    if (e0 >= m2) return;
+    int lo = 0, hi = n;  // source of e0
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if ((long long)xadj[mid] <= e0) lo = mid; else hi = mid;
+    }
+    int u = lo;
+    long long next = xadj[u + 1];
+    const long long e1 = e0 + 8 < m2 ? e0 + 8 : m2;
+    for (long long e = e0; e < e1; ++e) {
+        while (e >= next) {
+            ++u;
+            next = xadj[u + 1];
+        }
+        const int v = adjncy[e];
+        const float du = (float)(xadj[u + 1] - xadj[u]), dv = (float)(xadj[v + 1] - xadj[v]);
+        const float w = (float)adjwgt[e];
+        const float cu = (float)vwgt[u], cv = (float)vwgt[v];
+        const float cn = (float)__popcll(sig[u] & sig[v]);
+        const float agree = (float)((l0[u] == l0[v]) + (l1[u] == l1[v]) + (l2[u] == l2[v]));
+        const float es2 = (w * w) / fmaxf(cu * cv, 1.0f);
+        float* o = out + e * kFeatures;
+        o[0] = lg(fminf(du, dv));
+        o[1] = lg(fmaxf(du, dv));
+        o[2] = lg(w);
+        o[3] = lg(fminf(cu, cv));
+        o[4] = lg(fmaxf(cu, cv));
+        o[5] = cn * 0.125f;
+        o[6] = agree;
+        o[7] = lg(es2);
+    }
+}
+
+__global__ void expansion_star2_kernel(int n, long long m2, const int* __restrict__ xadj,
+                                       const int* __restrict__ adjncy,
+                                       const int* __restrict__ adjwgt,
+                                       const int* __restrict__ vwgt, int shift,
+                                       int* __restrict__ score) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long e0 = t * 8;
+    if (e0 >= m2) return;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if ((long long)xadj[mid] <= e0) lo = mid; else hi = mid;
+    }
+    int u = lo;
+    long long next = xadj[u + 1];
+    const long long e1 = e0 + 8 < m2 ? e0 + 8 : m2;
+    for (long long e = e0; e < e1; ++e) {
+        while (e >= next) {
+            ++u;
+            next = xadj[u + 1];
+        }
+        const long long w = adjwgt[e];
+        const long long c = (long long)vwgt[u] * (long long)vwgt[adjncy[e]];
+        // round(1024 w^2 / c) in integers
+        const long long num = 1024 * w * w;
+        long long q = c > 0 ? (num + c / 2) / c : (long long)kScoreMax;
+        if (q < 1) q = 1;
+        if (q > kScoreMax) q = kScoreMax;
+        score[e] = max((int)q >> shift, 1);
+    }
+}
+
+__global__ void quantize_kernel(long long m, const float* __restrict__ logit, int shift,
+                                int* __restrict__ score) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) score[i] = max(quantize_logit(logit[i]) >> shift, 1);
+}
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
+
+__global__ void max_degree_kernel(int n, const int* __restrict__ xadj, int* __restrict__ out) {
+    __shared__ int best;
+    if (threadIdx.x == 0) best = 0;
+    __syncthreads();
+    int d = 0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        d = max(d, xadj[v + 1] - xadj[v]);
+    atomicMax(&best, d);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, best);
+}
+
+// ------------------------------------------------------------------ MLP forward on MFMA
+// Packed weights (bf16 bits), one 8-element vector per lane:
+//   A1[t][lane]        t = 0..3 (hidden rows 16t..16t+15): W1[16t + lane%16][8g + j], g = lane/16,
+//                      zero for 8g + j >= 8
+//   A2[t2][kb][lane]   W2[16 t2 + lane%16][hid(kb, g, j)],
+//                      hid(kb, g, j) = 16 (2 kb + (j >= 4)) + 4 g + (j & 3)
+// followed by b1[64], b2[64], w3[64], b3 as fp32 (stored in the float tail of the same buffer).
+constexpr int kPackA1 = 4 * 64 * 8;
+constexpr int kPackA2 = 4 * 2 * 64 * 8;
+constexpr int kPackBf16 = kPackA1 + kPackA2;          // bf16 elements
+constexpr int kPackFloats = 64 + 64 + 64 + 1;        // fp32 tail
+constexpr int kPackBytes = kPackBf16 * 2 + kPackFloats * 4;
+
+__host__ __device__ inline int hid_of(int kb, int g, int j) {
+    return 16 * (2 * kb + (j >= 4 ? 1 : 0)) + 4 * g + (j & 3);
+}
+
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ feat,
+                                                          long long m,
+                                                          const uint16_t* __restrict__ pack,
+                                                          float* __restrict__ logit) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, col = lane & 15;
+    const bf16x8* A = reinterpret_cast<const bf16x8*>(pack);
+    bf16x8 a1[4], a2[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a1[t] = A[t * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) a2[t][kb] = A[kPackA1 / 8 + (t * 2 + kb) * 64 + lane];
+    const float* tail = reinterpret_cast<const float*>(pack + kPackBf16);
+    float b1[4][4], b2[4][4], w3[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            b1[t][r] = tail[16 * t + 4 * g + r];
+            b2[t][r] = tail[64 + 16 * t + 4 * g + r];
+            w3[t][r] = tail[128 + 16 * t + 4 * g + r];
+        }
+    const float b3 = tail[192];
+    const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (long long base = wave * 16; base < m; base += waves * 16) {
+        const long long e = base + col;
+        // B operand of layer 1: the 8 features of edge `col` in lanes of group 0
+        bf16x8 xb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float f = (g == 0 && e < m) ? feat[e * kFeatures + j] : 0.0f;
+            const uint16_t bits = f2bf(f);
+            __bf16 h;
+            memcpy(&h, &bits, 2);
+            xb[j] = h;
+        }
+        f32x4 h1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 z = {0.f, 0.f, 0.f, 0.f};
+            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], xb, z, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h1[t][r] = fmaxf(z[r] + b1[t][r], 0.0f);
+        }
+        f32x4 h2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h2[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            bf16x8 hb;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint16_t bits = f2bf(h1[2 * kb + (j >> 2)][j & 3]);
+                __bf16 h;
+                memcpy(&h, &bits, 2);
+                hb[j] = h;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                h2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t][kb], hb, h2[t], 0, 0, 0);
+        }
+        float acc = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc += w3[t][r] * fmaxf(h2[t][r] + b2[t][r], 0.0f);
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        if (g == 0 && e < m) logit[e] = acc + b3;
+    }
+}
+
+// ------------------------------------------------------------------ training (FP32 VALU)
+// One workgroup of 256 threads per 64 examples; forward with the same bf16 roundings as the
+// MFMA path (inputs, weights, layer-1 activations), backward in fp32 with straight-through
+// roundings. Per-workgroup gradients go to `partial` and are summed in a fixed order.
+constexpr int kTB = 64;  // examples per workgroup
+
+__global__ __launch_bounds__(256) void train_kernel(const float* __restrict__ feat,
+                                                    const float* __restrict__ label, int batch,
+                                                    const float* __restrict__ params,
+                                                    float* __restrict__ partial,
+                                                    float* __restrict__ loss_out) {
+    __shared__ float X[kTB][kFeatures];
+    __shared__ float W1[kHidden][kFeatures];
+    __shared__ float W2[kHidden][kHidden + 1];
+    __shared__ float Z1[kTB][kHidden + 1];
+    __shared__ float H1[kTB][kHidden + 1];
+    __shared__ float Z2[kTB][kHidden + 1];
+    __shared__ float D2[kTB][kHidden + 1];
+    __shared__ float dz3[kTB];
+    __shared__ float lsum[kTB];
+    const int tid = threadIdx.x;
+    const int e0 = blockIdx.x * kTB;
+    const int nb = min(kTB, batch - e0);
+    for (int i = tid; i < kTB * kFeatures; i += 256) {
+        const int e = i / kFeatures, f = i % kFeatures;
+        X[e][f] = e < nb ? round_bf(feat[(long long)(e0 + e) * kFeatures + f]) : 0.f;
+    }
+    for (int i = tid; i < kHidden * kFeatures; i += 256)
+        W1[i / kFeatures][i % kFeatures] = round_bf(params[kOffW1 + i]);
+    for (int i = tid; i < kHidden * kHidden; i += 256)
+        W2[i / kHidden][i % kHidden] = round_bf(params[kOffW2 + i]);
+    __syncthreads();
+    // layer 1
+    for (int i = tid; i < kTB * kHidden; i += 256) {
+        const int e = i / kHidden, h = i % kHidden;
+        float z = params[kOffB1 + h];
+        for (int f = 0; f < kFeatures; ++f) z += W1[h][f] * X[e][f];
+        Z1[e][h] = z;
+        H1[e][h] = round_bf(fmaxf(z, 0.f));
+    }
+    __syncthreads();
+    // layer 2
+    for (int i = tid; i < kTB * kHidden; i += 256) {
+        const int e = i / kHidden, h = i % kHidden;
+        float z = params[kOffB2 + h];
+        for (int k = 0; k < kHidden; ++k) z += W2[h][k] * H1[e][k];
+        Z2[e][h] = z;
+    }
+    __syncthreads();
+    // layer 3, loss, dz3
+    if (tid < kTB) {
+        const int e = tid;
+        float z = params[kOffB3];
+        for (int h = 0; h < kHidden; ++h) z += params[kOffW3 + h] * fmaxf(Z2[e][h], 0.f);
+        float l = 0.f, d = 0.f;
+        if (e < nb) {
+            const float y = label[e0 + e];
+            // BCE with logits, stable form
+            l = fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z)));
+            d = (1.f / (1.f + expf(-z)) - y) / (float)batch;
+        }
+        dz3[e] = d;
+        lsum[e] = l;
+    }
+    __syncthreads();
+    // dZ2 = dz3 * w3 * (Z2 > 0)
+    for (int i = tid; i < kTB * kHidden; i += 256) {
+        const int e = i / kHidden, h = i % kHidden;
+        D2[e][h] = Z2[e][h] > 0.f ? dz3[e] * params[kOffW3 + h] : 0.f;
+    }
+    __syncthreads();
+    float* out = partial + (long long)blockIdx.x * kMlpParams;
+    // dW3, db3
+    if (tid < kHidden) {
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += dz3[e] * fmaxf(Z2[e][tid], 0.f);
+        out[kOffW3 + tid] = s;
+    }
+    if (tid == 64) {
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += dz3[e];
+        out[kOffB3] = s;
+        float l = 0.f;
+        for (int e = 0; e < kTB; ++e) l += lsum[e];
+        loss_out[blockIdx.x] = l;
+    }
+    // dW2, db2
+    for (int i = tid; i < kHidden * kHidden; i += 256) {
+        const int h = i / kHidden, k = i % kHidden;
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += D2[e][h] * H1[e][k];
+        out[kOffW2 + i] = s;
+    }
+    if (tid < kHidden) {
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += D2[e][tid];
+        out[kOffB2 + tid] = s;
+    }
+    __syncthreads();
+    // dZ1 = (W2^T dZ2) * (Z1 > 0), reusing Z1 as storage
+    for (int i = tid; i < kTB * kHidden; i += 256) {
+        const int e = i / kHidden, k = i % kHidden;
+        float s = 0.f;
+        for (int h = 0; h < kHidden; ++h) s += W2[h][k] * D2[e][h];
+        H1[e][k] = Z1[e][k] > 0.f ? s : 0.f;
+    }
+    __syncthreads();
+    // dW1, db1
+    for (int i = tid; i < kHidden * kFeatures; i += 256) {
+        const int h = i / kFeatures, f = i % kFeatures;
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += H1[e][h] * X[e][f];
+        out[kOffW1 + i] = s;
+    }
+    if (tid < kHidden) {
+        float s = 0.f;
+        for (int e = 0; e < kTB; ++e) s += H1[e][tid];
+        out[kOffB1 + tid] = s;
+    }
+}
+
+__global__ void reduce_grads(int blocks, int batch, const float* __restrict__ partial,
+                             const float* __restrict__ block_loss, float* __restrict__ grads,
+                             float* __restrict__ loss) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kMlpParams) {
+        float s = 0.f;
+        for (int b = 0; b < blocks; ++b) s += partial[(long long)b * kMlpParams + p];
+        grads[p] = s;
+    } else if (p == kMlpParams && loss) {
+        float s = 0.f;
+        for (int b = 0; b < blocks; ++b) s += block_loss[b];
+        *loss = s / (float)batch;
+    }
+}
+
+__global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m,
+                            float* __restrict__ v, const float* __restrict__ g, float lr,
+                            float c1, float c2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    m[p] = b1 * m[p] + (1.f - b1) * g[p];
+    v[p] = b2 * v[p] + (1.f - b2) * g[p] * g[p];
+    params[p] -= lr * (m[p] / c1) / (sqrtf(v[p] / c2) + eps);
+}
+
+void pack_weights(const MlpWeights& w, std::vector<uint8_t>& out) {
+    out.assign(kPackBytes, 0);
+    uint16_t* bf = reinterpret_cast<uint16_t*>(out.data());
+    for (int t = 0; t < 4; ++t)
+        for (int lane = 0; lane < 64; ++lane) {
+            const int g = lane >> 4, row = lane & 15;
+            for (int j = 0; j < 8; ++j) {
+                const int k = 8 * g + j;
+                const float x = k < kFeatures ? w.p[kOffW1 + (16 * t + row) * kFeatures + k] : 0.f;
+                bf[(t * 64 + lane) * 8 + j] = f2bf(x);
+            }
+        }
+    for (int t = 0; t < 4; ++t)
+        for (int kb = 0; kb < 2; ++kb)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int g = lane >> 4, row = lane & 15;
+                for (int j = 0; j < 8; ++j) {
+                    const float x = w.p[kOffW2 + (16 * t + row) * kHidden + hid_of(kb, g, j)];
+                    bf[kPackA1 + ((t * 2 + kb) * 64 + lane) * 8 + j] = f2bf(x);
+                }
+            }
+    float* tail = reinterpret_cast<float*>(bf + kPackBf16);
+    for (int h = 0; h < kHidden; ++h) {
+        tail[h] = w.p[kOffB1 + h];
+        tail[64 + h] = w.p[kOffB2 + h];
+        tail[128 + h] = w.p[kOffW3 + h];
+    }
+    tail[192] = w.p[kOffB3];
+}
+
+}  // namespace
+
+void RatingWorkspace::features(const DGraph& g, int bound, uint32_t seed, float* out,
+                               hipStream_t s, LpWorkspace& lp, Profiler* prof) {
+    ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
+    sig_.reserve(g.n);
+    lab_.reserve(3LL * g.n);
+    size_.reserve(g.n);
+    if (g.n > 0)
+        hipLaunchKernelGGL(sketch_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
+                           g.adjncy, sig_.p);
+    BinConfig bins;
+    for (int i = 0; i < 3; ++i) {
+        int* lab = lab_.p + (long long)i * g.n;
+        const uint32_t sd = hash3(seed, 0xA6u, (uint32_t)i);
+        LpWorkspace::init_cluster_labels(g, lab, size_.p, s);
+        lp.prepare(LpMode::Cluster, g, g.n, 4, sd, bins, s);
+        lp.run(LpMode::Cluster, g, lab, size_.p, bound, 1, sd, s, nullptr, "rating_sweep");
+    }
+    if (g.m2 > 0)
+        hipLaunchKernelGGL(features_kernel, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256), 0, s,
+                           g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p, lab_.p,
+                           lab_.p + g.n, lab_.p + 2LL * g.n, out);
+    KH_CHECK(hipGetLastError());
+}
+
+void RatingWorkspace::mlp_forward(const float* feat, long long m, const MlpWeights& w,
+                                  float* logits, hipStream_t s, Profiler* prof) {
+    std::vector<uint8_t> host;
+    pack_weights(w, host);
+    wpack_.reserve(kPackBytes / 2);
+    KH_CHECK(hipMemcpyAsync(wpack_.p, host.data(), kPackBytes, hipMemcpyHostToDevice, s));
+    if (m > 0) {
+        // 9344 useful FLOP per edge (2 x (8*64 + 64*64 + 64))
+        ProfScope ps(prof, "mlp_forward", 9344.0 * (double)m, m, s);
+        const long long groups = (m + 15) / 16;
+        const int blocks = (int)std::min<long long>((groups + 3) / 4, 8192);
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(256), 0, s, feat, m, wpack_.p,
+                           logits);
+    }
+    KH_CHECK(hipStreamSynchronize(s));  // `host` must outlive the copy
+    KH_CHECK(hipGetLastError());
+}
+
+int RatingWorkspace::score_shift(const DGraph& g, hipStream_t s) {
+    map_.reserve(1);
+    int* d = map_.p;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, d);
+    hipLaunchKernelGGL(max_degree_kernel, dim3(std::min(grid_for(g.n, 256), 1024)), dim3(256), 0, s,
+                       g.n, g.xadj, d);
+    int maxdeg = 0;
+    KH_CHECK(hipMemcpyAsync(&maxdeg, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    int shift = 0;
+    while (shift < 12 && (long long)maxdeg * (kScoreMax >> shift) >= (1LL << 31)) ++shift;
+    KH_REQUIRE((long long)maxdeg * (kScoreMax >> shift) < (1LL << 31), "degree too large");
+    return shift;
+}
+
+void RatingWorkspace::scores(RatingMode mode, const DGraph& g, int bound, uint32_t seed,
+                             const MlpWeights* w, int* score, hipStream_t s, LpWorkspace& lp,
+                             Profiler* prof) {
+    if (g.m2 == 0) return;
+    const int shift = score_shift(g, s);
+    if (mode == RatingMode::ExpansionStar2) {
+        ProfScope ps(prof, "rating_expansion_star2", 0.0, g.m2, s);
+        hipLaunchKernelGGL(expansion_star2_kernel, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256),
+                           0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, shift, score);
+        KH_CHECK(hipGetLastError());
+        return;
+    }
+    KH_REQUIRE(mode == RatingMode::Learned && w, "learned rating needs MLP weights");
+    feat_.reserve(g.m2 * kFeatures);
+    logit_.reserve(g.m2);
+    features(g, bound, seed, feat_.p, s, lp, prof);
+    mlp_forward(feat_.p, g.m2, *w, logit_.p, s, prof);
+    hipLaunchKernelGGL(quantize_kernel, dim3(grid_for(g.m2, 256)), dim3(256), 0, s, g.m2, logit_.p,
+                       shift, score);
+    KH_CHECK(hipGetLastError());
+}
+
+void RatingWorkspace::train_step(const float* feat, const float* labels, int batch,
+                                 float* params, float* m, float* v, float* grads, int step,
+                                 float lr, float* loss, hipStream_t s) {
+    KH_REQUIRE(batch > 0, "empty batch");
+    const int blocks = (batch + kTB - 1) / kTB;
+    partial_.reserve((long long)blocks * kMlpParams);
+    loss_.reserve(blocks);
+    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, s, feat, labels, batch, params,
+                       partial_.p, loss_.p);
+    hipLaunchKernelGGL(reduce_grads, dim3(grid_for(kMlpParams + 1, 256)), dim3(256), 0, s, blocks,
+                       batch, partial_.p, loss_.p, grads, loss);
+    if (lr > 0) {
+        const float c1 = 1.f - std::pow(0.9f, (float)step);
+        const float c2 = 1.f - std::pow(0.999f, (float)step);
+        hipLaunchKernelGGL(adam_kernel, dim3(grid_for(kMlpParams, 256)), dim3(256), 0, s, params,
+                           m, v, grads, lr, c1, c2);
+    }
+    KH_CHECK(hipGetLastError());
+}
+
+}  // namespace kahip

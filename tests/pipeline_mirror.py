@@ -1,7 +1,8 @@
 """CPU mirror of the library's whole FASTSOCIAL pipeline, composed from the oracle's R2 steps
 (synchronous LP clustering and refinement, first-appearance relabel, contraction, projection)
 and the library's own CPU initial partitioner, with the library's per-level seeds:
-    cluster seed of level l = hash3(seed, 0xC1, l), refinement seed = hash3(seed, 0x4E, l).
+    cluster seed of level l = hash3(seed, 0xC1, l), refinement seed = hash3(seed, 0x4E, l),
+    rating seed = hash3(seed, 0x5A, l).
 The GPU pipeline must reproduce its partition bit for bit. Test infrastructure only."""
 from __future__ import annotations
 
@@ -25,9 +26,11 @@ def ip_seed(seed):
 
 def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, cluster_rounds=10,
                      refine_rounds=25, subrounds=4, cluster_factor=18, stop_factor=5000,
-                     ip=None):
+                     ip=None, scores=None, rating_levels=64):
     """Returns (part, info). `ip(xadj, adjncy, vwgt, adjwgt, k, U, reps, seed)` partitions the
-    coarsest graph (the library's kahip_amd.initial_partition by default)."""
+    coarsest graph (the library's kahip_amd.initial_partition by default). `scores(xadj, adjncy,
+    vwgt, adjwgt, B, rating_seed)` (optional) gives the int32 edge scores that replace the edge
+    weights in the clustering of levels < rating_levels."""
     if ip is None:
         import kahip_amd
         ip = kahip_amd.initial_partition
@@ -44,7 +47,10 @@ def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, clus
     while again:
         x, a, vw, ew = levels[level]
         cs = O.hash3(seed, 0xC1, level)
-        cl, nc, _ = O.lp_cluster_sync(x, a, vw, ew, B, cluster_rounds, subrounds, cs)
+        rw = ew
+        if scores is not None and level < rating_levels and int(x[-1]) > 0:
+            rw = np.asarray(scores(x, a, vw, ew, B, O.hash3(seed, 0x5A, level)), np.int32)
+        cl, nc, _ = O.lp_cluster_sync(x, a, vw, rw, B, cluster_rounds, subrounds, cs)
         cx, ca, cw, cv = O.contract(x, a, vw, ew, cl, nc)
         maps.append(cl)
         levels.append((cx, ca, cv, cw))
