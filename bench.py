@@ -11,7 +11,10 @@ cut, hipEvents on the library's stream). The CPU initial partitioning is outside
 time is in ms_per_step (wall clock per full step) and reported separately.
 
 roofline: the dominant kernel (largest share of the timed LP launches), with algorithmic bytes
-per launch (12 B per traversed edge + 16 B per node) over its hipEvent-measured duration.
+per launch (12 B per traversed edge + 16 B per node) over its duration, measured with hipEvents on
+the library's stream around every launch of the timed steps. `traffic` (HBM bytes per launch of
+that kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed as
+profiles/traffic_<workload>.json, when one exists for the same kernel.
 cpu_baseline: the reference's own LP clustering (compiled from its sources, oracle/_ref) timed on
 this host's cores on a bounded sample of the same graph (see the 'sample' field).
 """
@@ -64,6 +67,19 @@ def algorithmic_bytes(st, cluster_rounds=10):
     b += 12.0 * st["edges_refine"]
     b += 12.0 * st["edges_cut"] + 12.0 * st["level_n"][0]
     return b
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("profiler_scope") != kernel:
+        return None
+    return t.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(K, g, w, budget_s=30.0):
@@ -129,11 +145,18 @@ def main():
         _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False)
         log(f"warmup {i}: cut={st['cut']} gpu_ms={st['ms_coarsen'] + st['ms_refine'] + st['ms_cut']:.1f} "
             f"ip_ms={st['ms_initial']:.1f} levels={st['level_n']}")
+    warm_gpu_ms = (st["ms_coarsen"] + st["ms_refine"] + st["ms_cut"]) if args.warmup else None
     stats = []
+    prof = {}
     t_start = time.perf_counter()
     for i in range(args.steps):
-        _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False)
+        # every timed step records per-launch hipEvents (the roofline's kernel durations)
+        _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
         stats.append(st)
+        for name, v in K.profile().items():
+            acc = prof.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "edges": 0})
+            for key in acc:
+                acc[key] += v[key]
         log(f"step {i}: cut={st['cut']} balance={st['balance']:.4f} "
             f"coarsen={st['ms_coarsen']:.1f}ms refine={st['ms_refine']:.1f}ms "
             f"ip={st['ms_initial']:.1f}ms")
@@ -146,9 +169,6 @@ def main():
     path_bytes = algorithmic_bytes(stats[-1])
     path_gbs = path_bytes / (gpu_ms[-1] / 1e3) / 1e9
 
-    # one profiled step (untimed) for the per-kernel roofline
-    _, pst = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
-    prof = K.profile()
     timed = {k: v for k, v in prof.items() if v["bytes"] > 0}
     dom_name = max(timed, key=lambda k: timed[k]["ms"]) if timed else None
     roofline = None
@@ -158,7 +178,8 @@ def main():
         per_launch_ms = d["ms"] / d["launches"]
         achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.workload, dom_name),
                     "kernel": dom_name, "launches": d["launches"],
                     "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_ms * 1e3, 2)}
     lp_all = [v for k, v in prof.items() if k.startswith("lp_cluster_propose")]
@@ -193,6 +214,7 @@ def main():
                    "n": info["n"], "m2": info["m2"], "k": w["k"], "imbalance": 0.03,
                    "parallelism": "single-gpu"},
         "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
+        "gpu_ms_unprofiled_warmup": None if warm_gpu_ms is None else round(warm_gpu_ms, 2),
         "cpu_initial_partition_ms_per_step": round(sum(s["ms_initial"] for s in stats) / len(stats), 1),
         "path_hbm_gbs": round(path_gbs, 1),
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 4),

@@ -81,6 +81,10 @@ int kahip_graph_from_host(int n, const int* xadj, const int* adjncy, const int* 
  * kahip_amd/graphs.py. */
 int kahip_graph_rmat(int scale, long long samples, unsigned long long seed, double a, double b,
                      double c, kahip_graph_t* out);
+/* Random geometric graph generated on the GPU: n points uniform in [0,1)^2, an edge between two
+ * points at distance < radius (radius <= 0: 0.55 sqrt(ln n / n)), unit weights, nodes in
+ * grid-cell order, rows sorted; the same graph as kahip_amd/graphs.py gen_rgg. */
+int kahip_graph_rgg(int n, unsigned long long seed, double radius, kahip_graph_t* out);
 int kahip_graph_info(kahip_graph_t g, int* n, long long* m2, long long* total_vwgt,
                      int* max_degree);
 int kahip_graph_to_host(kahip_graph_t g, int* xadj, int* adjncy, int* vwgt, int* adjwgt);
@@ -144,6 +148,49 @@ int kahip_mlp_gradient(int count, const float* features, const float* labels,
  * each pass's mean of batch losses. Deterministic. */
 int kahip_mlp_train(long long count, const float* features, const float* labels, int batch,
                     int epochs, float lr, float* params, double* loss_per_epoch);
+
+/* ----------------------------------------------------------------------------------------
+ * Batched mode (no reference counterpart: the reference partitions one graph per call,
+ * interface/kaHIP_interface.cpp:66-166). G independent graphs live in HBM as one disjoint union
+ * (node ids of graph g offset by node_off[g]); each call partitions every graph into k blocks
+ * with its own bounds (U_g, B_g from its own weight) and its own stop rule, sharing every GPU
+ * launch. Per-graph results equal the union's synchronous schedule with per-label bounds, which
+ * the oracle restates (orc_lp_cluster_sync_ex / orc_lp_refine_sync_ex). */
+typedef struct kahip_batch* kahip_batch_t;
+
+typedef struct {
+    double ms_coarsen, ms_initial, ms_refine, ms_cut;
+    int levels;
+    long long edges_cluster, edges_contract, edges_refine, edges_cut;
+} kahip_batch_stats;
+
+/* xadj: the graphs' local CSR offsets concatenated (node_off[G] + G entries, graph g's at
+ * xadj[node_off[g] + g ...]); adjncy: local neighbour ids concatenated (edge_off[G] entries);
+ * vwgt / adjwgt concatenated or NULL. Every graph needs at least one node. */
+int kahip_batch_from_host(int num_graphs, const long long* node_off, const long long* edge_off,
+                          const int* xadj, const int* adjncy, const int* vwgt, const int* adjwgt,
+                          kahip_batch_t* out);
+/* Graph i: RGG (even i) with n = floor(2^x + 0.5) nodes or R-MAT (odd i) of scale
+ * floor(x + 0.5) with 16 samples per node, x = log2_min + u (log2_max - log2_min), u and the
+ * graph's seed drawn from `seed` (kahip_amd/graphs.py batch_spec restates it). */
+int kahip_batch_generate(int num_graphs, double log2_min, double log2_max, unsigned long long seed,
+                         kahip_batch_t* out);
+int kahip_batch_info(kahip_batch_t b, int* num_graphs, int* n, long long* m2, int* node_off);
+/* The union CSR (global ids). */
+int kahip_batch_to_host(kahip_batch_t b, int* xadj, int* adjncy, int* vwgt, int* adjwgt);
+void kahip_batch_free(kahip_batch_t b);
+/* part: n local block ids (may be NULL); cuts / max_block_weights: G values (may be NULL).
+ * use_graph != 0 runs every LP round loop as one captured hipGraph (same results). */
+int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_graph, int* part,
+                          long long* cuts, long long* max_block_weights,
+                          kahip_batch_stats* stats);
+/* Autotuner of the LP degree bins for a device graph or batch (exactly one non-NULL): times two
+ * LP clustering rounds per candidate upper bound of the wavefront bin (bins[4] in 96..512) and
+ * returns the fastest bins; results of the LP do not depend on the bins. ms_per_candidate
+ * (may be NULL) receives KAHIP_AUTOTUNE_CANDIDATES timings. */
+#define KAHIP_AUTOTUNE_CANDIDATES 6
+int kahip_autotune_bins(kahip_graph_t g, kahip_batch_t b, const kahip_params* params, int* bins,
+                        double* ms_per_candidate);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,16 @@ class Params(C.Structure):
                 ("mlp_weights", C.POINTER(C.c_float))]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("ms_coarsen", C.c_double), ("ms_initial", C.c_double),
+                ("ms_refine", C.c_double), ("ms_cut", C.c_double), ("levels", C.c_int),
+                ("edges_cluster", C.c_longlong), ("edges_contract", C.c_longlong),
+                ("edges_refine", C.c_longlong), ("edges_cut", C.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("cut", C.c_longlong), ("balance", C.c_double), ("upper_bound", C.c_uint),
                 ("cluster_bound", C.c_uint), ("levels", C.c_int),
@@ -79,11 +89,12 @@ class Stats(C.Structure):
 EXPORTS = [
     "kaffpa", "kaffpa_balance", "kaffpa_balance_NE", "kahip_default_params", "kahip_last_error",
     "kaffpa_gpu_ex", "kahip_graph_from_host", "kahip_graph_rmat", "kahip_graph_info",
-    "kahip_graph_to_host", "kahip_graph_free", "kahip_partition", "kahip_profile_json",
+    "kahip_graph_to_host", "kahip_graph_free", "kahip_graph_rgg", "kahip_partition", "kahip_profile_json",
     "kahip_lp_cluster", "kahip_remap", "kahip_contract", "kahip_lp_refine", "kahip_edge_cut",
     "kahip_block_weights", "kahip_project", "kahip_initial_partition", "kahip_edge_features",
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
-    "kahip_mlp_train",
+    "kahip_mlp_train", "kahip_batch_from_host", "kahip_batch_generate", "kahip_batch_info",
+    "kahip_batch_to_host", "kahip_batch_free", "kahip_batch_partition", "kahip_autotune_bins",
 ]
 
 _lib = None
@@ -121,6 +132,7 @@ def lib():
     L.kahip_graph_from_host.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.POINTER(G)]
     L.kahip_graph_rmat.argtypes = [C.c_int, C.c_longlong, C.c_ulonglong, C.c_double,
                                    C.c_double, C.c_double, C.POINTER(G)]
+    L.kahip_graph_rgg.argtypes = [C.c_int, C.c_ulonglong, C.c_double, C.POINTER(G)]
     L.kahip_graph_info.argtypes = [G, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
     L.kahip_graph_to_host.argtypes = [G, ip, ip, ip, ip]
     L.kahip_graph_free.argtypes = [G]
@@ -148,6 +160,18 @@ def lib():
                                      C.POINTER(C.c_double)]
     L.kahip_mlp_train.argtypes = [C.c_longlong, _f32p, _f32p, C.c_int, C.c_int, C.c_float,
                                   _f32p, _f64p]
+    L.kahip_batch_from_host.argtypes = [C.c_int, _i64p, _i64p, _i32p, _i32p, ip, ip,
+                                        C.POINTER(G)]
+    L.kahip_batch_generate.argtypes = [C.c_int, C.c_double, C.c_double, C.c_ulonglong,
+                                       C.POINTER(G)]
+    L.kahip_batch_info.argtypes = [G, ip, ip, C.POINTER(C.c_longlong), ip]
+    L.kahip_batch_to_host.argtypes = [G, ip, ip, ip, ip]
+    L.kahip_batch_free.argtypes = [G]
+    L.kahip_batch_free.restype = None
+    L.kahip_batch_partition.argtypes = [G, C.POINTER(Params), C.c_int, ip,
+                                        C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                        C.POINTER(BatchStats)]
+    L.kahip_autotune_bins.argtypes = [G, G, C.POINTER(Params), ip, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -247,6 +271,13 @@ class Graph:
     def rmat(cls, scale, samples, seed, a=0.57, b=0.19, c=0.19):
         h = C.c_void_p()
         _check(lib().kahip_graph_rmat(scale, samples, seed, a, b, c, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rgg(cls, n, seed, radius=0.0):
+        """Random geometric graph generated on the GPU (equal to graphs.gen_rgg(n, seed))."""
+        h = C.c_void_p()
+        _check(lib().kahip_graph_rgg(int(n), int(seed), float(radius), C.byref(h)))
         return cls(h)
 
     def info(self):
@@ -455,3 +486,98 @@ def load_rating_weights(path=None):
     if w.size != MLP_PARAMS:
         raise KahipError(f"{path}: expected {MLP_PARAMS} floats")
     return w
+
+
+# ----------------------------------------------------------------------------- batched mode
+class Batch:
+    """G independent graphs resident in GPU memory as one disjoint union."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_graphs(cls, graphs):
+        """graphs: list of (xadj, adjncy, vwgt, adjwgt) with local ids (weights may be None)."""
+        G = len(graphs)
+        node_off = np.zeros(G + 1, np.int64)
+        edge_off = np.zeros(G + 1, np.int64)
+        for i, (x, a, _, _) in enumerate(graphs):
+            node_off[i + 1] = node_off[i] + len(x) - 1
+            edge_off[i + 1] = edge_off[i] + int(x[-1])
+        xs = np.concatenate([_i32(x) for x, _, _, _ in graphs])
+        adj = np.concatenate([_i32(a) for _, a, _, _ in graphs] + [np.zeros(1, np.int32)])
+        any_vw = any(v is not None for _, _, v, _ in graphs)
+        any_ew = any(w is not None for _, _, _, w in graphs)
+        vw = np.concatenate([_i32(v) if v is not None else np.ones(len(x) - 1, np.int32)
+                             for x, _, v, _ in graphs]) if any_vw else None
+        ew = np.concatenate([_i32(w) if w is not None else np.ones(int(x[-1]), np.int32)
+                             for x, _, _, w in graphs]) if any_ew else None
+        h = C.c_void_p()
+        _check(lib().kahip_batch_from_host(G, node_off, edge_off, xs, adj, _p(vw), _p(ew),
+                                           C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def generate(cls, num_graphs, log2_min=10.0, log2_max=16.0, seed=0):
+        h = C.c_void_p()
+        _check(lib().kahip_batch_generate(num_graphs, log2_min, log2_max, seed, C.byref(h)))
+        return cls(h)
+
+    def info(self):
+        G, n = C.c_int(), C.c_int()
+        m2 = C.c_longlong()
+        _check(lib().kahip_batch_info(self._h, C.byref(G), C.byref(n), C.byref(m2), None))
+        off = np.zeros(G.value + 1, np.int32)
+        _check(lib().kahip_batch_info(self._h, None, None, None,
+                                      off.ctypes.data_as(C.POINTER(C.c_int))))
+        return {"graphs": G.value, "n": n.value, "m2": m2.value, "node_off": off}
+
+    def to_host(self):
+        inf = self.info()
+        ip = C.POINTER(C.c_int)
+        x = np.zeros(inf["n"] + 1, np.int32)
+        a = np.zeros(max(inf["m2"], 1), np.int32)
+        w = np.zeros(max(inf["m2"], 1), np.int32)
+        v = np.zeros(max(inf["n"], 1), np.int32)
+        _check(lib().kahip_batch_to_host(self._h, x.ctypes.data_as(ip), a.ctypes.data_as(ip),
+                                         v.ctypes.data_as(ip), w.ctypes.data_as(ip)))
+        return x, a[:inf["m2"]], v[:inf["n"]], w[:inf["m2"]]
+
+    def partition(self, k, imbalance=0.03, seed=0, use_graph=False, **params):
+        """Returns (local part ndarray, cuts[G], max block weights[G], stats dict)."""
+        inf = self.info()
+        G = inf["graphs"]
+        part = np.zeros(max(inf["n"], 1), np.int32)
+        cuts = np.zeros(G, np.int64)
+        mbw = np.zeros(G, np.int64)
+        st = BatchStats()
+        p = default_params(k=k, imbalance=imbalance, seed=seed, **params)
+        LL = C.POINTER(C.c_longlong)
+        _check(lib().kahip_batch_partition(self._h, C.byref(p), 1 if use_graph else 0,
+                                           part.ctypes.data_as(C.POINTER(C.c_int)),
+                                           cuts.ctypes.data_as(LL), mbw.ctypes.data_as(LL),
+                                           C.byref(st)))
+        return part[:inf["n"]], cuts, mbw, st.as_dict()
+
+    def free(self):
+        if self._h:
+            lib().kahip_batch_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def autotune_bins(target, k=8, **params):
+    """Autotuned LP bins for a Graph or Batch: (bins list, ms per candidate)."""
+    bins = np.zeros(6, np.int32)
+    ms = np.zeros(6, np.float64)
+    p = default_params(k=k, **params)
+    g = target._h if isinstance(target, Graph) else None
+    b = target._h if isinstance(target, Batch) else None
+    _check(lib().kahip_autotune_bins(g, b, C.byref(p), bins.ctypes.data_as(C.POINTER(C.c_int)),
+                                     ms.ctypes.data_as(C.POINTER(C.c_double))))
+    return bins.tolist(), ms.tolist()

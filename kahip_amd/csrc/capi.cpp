@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "driver.hpp"
 #include "gen.hpp"
 #include "kahip_amd.h"
@@ -20,6 +21,11 @@ using namespace kahip;
 
 struct kahip_graph {
     OwnedGraph og;
+};
+
+struct kahip_batch {
+    OwnedGraph og;
+    std::vector<int> node_off;
 };
 
 namespace {
@@ -322,6 +328,19 @@ int kahip_graph_rmat(int scale, long long samples, unsigned long long seed, doub
         auto* g = new kahip_graph();
         try {
             generate_rmat(scale, samples, seed, a, b, c, g->og);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int kahip_graph_rgg(int n, unsigned long long seed, double radius, kahip_graph_t* out) {
+    return guarded([&] {
+        auto* g = new kahip_graph();
+        try {
+            generate_rgg(n, seed, radius, g->og);
         } catch (...) {
             delete g;
             throw;
@@ -702,6 +721,114 @@ int kahip_mlp_train(long long count, const float* features, const float* labels,
         }
         KH_CHECK(hipMemcpyAsync(params, w.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kahip_batch_from_host(int num_graphs, const long long* node_off, const long long* edge_off,
+                          const int* xadj, const int* adjncy, const int* vwgt, const int* adjwgt,
+                          kahip_batch_t* out) {
+    return guarded([&] {
+        KH_REQUIRE(num_graphs >= 1 && node_off && edge_off && xadj, "bad batch arguments");
+        KH_REQUIRE(node_off[0] == 0 && edge_off[0] == 0, "batch offsets must start at 0");
+        for (int g = 0; g < num_graphs; ++g)
+            KH_REQUIRE(node_off[g + 1] > node_off[g] && edge_off[g + 1] >= edge_off[g],
+                       "batch offsets must increase (every graph needs a node)");
+        auto* b = new kahip_batch();
+        try {
+            hipStream_t s;
+            KH_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            upload_batch(num_graphs, node_off, edge_off, xadj, adjncy, vwgt, adjwgt, b->og, s);
+            (void)hipStreamDestroy(s);
+            b->node_off.assign(num_graphs + 1, 0);
+            for (int g = 0; g <= num_graphs; ++g) b->node_off[g] = (int)node_off[g];
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int kahip_batch_generate(int num_graphs, double log2_min, double log2_max, unsigned long long seed,
+                         kahip_batch_t* out) {
+    return guarded([&] {
+        auto* b = new kahip_batch();
+        try {
+            generate_batch(num_graphs, log2_min, log2_max, seed, b->og, b->node_off);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int kahip_batch_info(kahip_batch_t b, int* num_graphs, int* n, long long* m2, int* node_off) {
+    return guarded([&] {
+        KH_REQUIRE(b, "null batch");
+        const int G = (int)b->node_off.size() - 1;
+        if (num_graphs) *num_graphs = G;
+        if (n) *n = b->og.g.n;
+        if (m2) *m2 = b->og.g.m2;
+        if (node_off) std::copy(b->node_off.begin(), b->node_off.end(), node_off);
+    });
+}
+
+int kahip_batch_to_host(kahip_batch_t b, int* xadj, int* adjncy, int* vwgt, int* adjwgt) {
+    return guarded([&] {
+        KH_REQUIRE(b, "null batch");
+        const DGraph& d = b->og.g;
+        if (xadj) KH_CHECK(hipMemcpy(xadj, d.xadj, sizeof(int) * (d.n + 1), hipMemcpyDeviceToHost));
+        if (adjncy && d.m2) KH_CHECK(hipMemcpy(adjncy, d.adjncy, sizeof(int) * d.m2, hipMemcpyDeviceToHost));
+        if (adjwgt && d.m2) KH_CHECK(hipMemcpy(adjwgt, d.adjwgt, sizeof(int) * d.m2, hipMemcpyDeviceToHost));
+        if (vwgt && d.n) KH_CHECK(hipMemcpy(vwgt, d.vwgt, sizeof(int) * d.n, hipMemcpyDeviceToHost));
+    });
+}
+
+void kahip_batch_free(kahip_batch_t b) { delete b; }
+
+int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_graph, int* part,
+                          long long* cuts, long long* max_block_weights,
+                          kahip_batch_stats* stats) {
+    return guarded([&] {
+        KH_REQUIRE(b, "null batch");
+        PartitionParams p = to_params(params);
+        Engine eng;
+        std::vector<int> tmp;
+        if (!part) {
+            tmp.resize(std::max(b->og.g.n, 1));
+            part = tmp.data();
+        }
+        BatchStats st;
+        const int G = (int)b->node_off.size() - 1;
+        partition_batch(eng, b->og.g, G, b->node_off.data(), p, part, &st, use_graph != 0);
+        for (int g = 0; g < G; ++g) {
+            if (cuts) cuts[g] = st.cut[g];
+            if (max_block_weights) max_block_weights[g] = st.max_block_weight[g];
+        }
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->ms_coarsen = st.ms_coarsen;
+            stats->ms_initial = st.ms_initial;
+            stats->ms_refine = st.ms_refine;
+            stats->ms_cut = st.ms_cut;
+            stats->levels = st.levels;
+            stats->edges_cluster = st.edges_cluster;
+            stats->edges_contract = st.edges_contract;
+            stats->edges_refine = st.edges_refine;
+            stats->edges_cut = st.edges_cut;
+        }
+    });
+}
+
+int kahip_autotune_bins(kahip_graph_t g, kahip_batch_t b, const kahip_params* params, int* bins,
+                        double* ms_per_candidate) {
+    return guarded([&] {
+        KH_REQUIRE((g != nullptr) != (b != nullptr), "pass exactly one of graph and batch");
+        PartitionParams p = to_params(params);
+        const DGraph& d = g ? g->og.g : b->og.g;
+        BinConfig best = autotune_bins(d, p, ms_per_candidate);
+        for (int i = 0; i < kNumBins; ++i) bins[i] = best.hi[i];
     });
 }
 

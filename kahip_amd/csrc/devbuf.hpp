@@ -118,8 +118,11 @@ struct ProfScope {
     ProfScope(Profiler* prof, const std::string& name, double bytes, long long edges,
               hipStream_t st)
         : p(prof), idx(prof ? prof->begin(name, bytes, edges, st) : -1), s(st) {}
-    ~ProfScope() {
+    ~ProfScope() { close(); }
+    // Ends the timed range early (idempotent).
+    void close() {
         if (p) p->end(idx, s);
+        p = nullptr;
     }
 };
 

@@ -61,6 +61,10 @@ __device__ inline int bucket_of(int lab, int P) {
     return (int)(mix32((uint32_t)lab ^ 0x5bd1e995u) % (uint32_t)P);
 }
 
+__device__ inline int bound_of(const LpArgs& a, int lab) {
+    return a.lbound ? a.lbound[lab] : a.bound;
+}
+
 __device__ inline bool better(int ac, uint32_t at, int al, int bc, uint32_t bt, int bl) {
     return ac > bc || (ac == bc && (at > bt || (at == bt && al < bl)));
 }
@@ -100,7 +104,7 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
     if (M == LpMode::Cluster) {
         go = c.conn > conn_own || (c.conn == conn_own && c.size >= size_own);
     } else {
-        overloaded = size_own > a.bound;
+        overloaded = size_own > bound_of(a, own);
         go = c.conn > conn_own || overloaded ||
              (c.conn == conn_own && (long long)c.size + 2LL * w <= (long long)size_own);
     }
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     Cand c{-1, 0u, 0x7fffffff, 0};
     if (leader && lab != own) {
         const int sz = a.size[lab];
-        if ((long long)sz + nw < (long long)a.bound) {
+        if ((long long)sz + nw < (long long)bound_of(a, lab)) {
             c.conn = conn;
             c.tie = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
             c.lab = lab;
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
                 continue;
             }
             const int sz = a.size[lab];
-            if ((long long)sz + nw >= (long long)a.bound) continue;
+            if ((long long)sz + nw >= (long long)bound_of(a, lab)) continue;
             const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
             if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
         }
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
             continue;
         }
         const int sz = a.size[b];
-        if ((long long)sz + nw >= (long long)a.bound) continue;
+        if ((long long)sz + nw >= (long long)bound_of(a, b)) continue;
         const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
         if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
     }
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
             continue;
         }
         const int sz = a.size[lab];
-        if ((long long)sz + nw >= (long long)a.bound) continue;
+        if ((long long)sz + nw >= (long long)bound_of(a, lab)) continue;
         const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
         if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
     }
@@ -557,7 +561,9 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
                 h = (h + 1) & (cap - 1);
             }
         }
-        __threadfence();
+        // Only the global table needs the device-scope fence (it waits for the atomics and
+        // invalidates the vector L1); it is costly, so the LDS path, the common one, skips it.
+        if (!in_lds) __threadfence();  // uniform per block
         __syncthreads();
         const int own = a.label[v];
         const int nw = a.vwgt[v];
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
                 continue;
             }
             const int sz = a.size[lab];
-            if ((long long)sz + nw >= (long long)a.bound) continue;
+            if ((long long)sz + nw >= (long long)bound_of(a, lab)) continue;
             const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
             if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
         }
@@ -638,10 +644,11 @@ __global__ void filter_departures(LpArgs a, const int* __restrict__ list, int co
     if (t < 0) return;
     const int own = a.label[v];
     const int so = a.size[own];
-    if (so <= a.bound) return;  // not a departure from an overloaded block
+    const int ub = bound_of(a, own);
+    if (so <= ub) return;  // not a departure from an overloaded block
     const int w = a.vwgt[v];
     const unsigned long long key = prio_key(a.prop_gain[v], v);
-    const bool pass = (long long)so - (long long)a.out[own] >= (long long)a.bound ||
+    const bool pass = (long long)so - (long long)a.out[own] >= (long long)ub ||
                       a.dtop[own] == key;
     if (pass) {
         atomicAdd(&a.inc[t], w);
@@ -659,8 +666,9 @@ __global__ void admit_moves(LpArgs a, const int* __restrict__ list, int count) {
     if (t < 0 || !a.prop_ok[v]) return;
     const int w = a.vwgt[v];
     const long long st = a.size[t];
-    const bool ok = st + (long long)a.inc[t] < (long long)a.bound ||
-                    (a.top[t] == prio_key(a.prop_gain[v], v) && st + w < (long long)a.bound);
+    const long long bt = bound_of(a, t);
+    const bool ok = st + (long long)a.inc[t] < bt ||
+                    (a.top[t] == prio_key(a.prop_gain[v], v) && st + w < bt);
     if (!ok) a.prop_ok[v] = 0;
 }
 
@@ -722,8 +730,8 @@ __device__ inline int bin_of(const BinConfig& cfg, int deg) {
 }
 
 __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, int S,
-                          BinConfig cfg, int* __restrict__ counts,
-                          unsigned long long* __restrict__ edges) {
+                          BinConfig cfg, const uint8_t* __restrict__ include,
+                          int* __restrict__ counts, unsigned long long* __restrict__ edges) {
     __shared__ int local[kMaxSub * kNumBins];
     __shared__ unsigned long long ledges[kMaxSub * kNumBins];
     for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) {
@@ -732,7 +740,7 @@ __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, in
     }
     __syncthreads();
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) {
+    if (v < n && (!include || include[v])) {
         const int deg = xadj[v + 1] - xadj[v];
         if (deg > 0) {
             const int l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
@@ -750,15 +758,16 @@ __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, in
 }
 
 __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, int S,
-                            BinConfig cfg, const int* __restrict__ offsets,
-                            int* __restrict__ cursor, int* __restrict__ lists) {
+                            BinConfig cfg, const uint8_t* __restrict__ include,
+                            const int* __restrict__ offsets, int* __restrict__ cursor,
+                            int* __restrict__ lists) {
     __shared__ int local[kMaxSub * kNumBins];
     __shared__ int base[kMaxSub * kNumBins];
     for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) local[i] = 0;
     __syncthreads();
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     int l = -1, r = 0;
-    if (v < n) {
+    if (v < n && (!include || include[v])) {
         const int deg = xadj[v + 1] - xadj[v];
         if (deg > 0) {
             l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
@@ -845,7 +854,7 @@ void fill_async(long long* p, long long n, long long value, hipStream_t s) {
 }
 
 void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, uint32_t seed,
-                          const BinConfig& bins, hipStream_t s) {
+                          const BinConfig& bins, hipStream_t s, const uint8_t* include) {
     KH_REQUIRE(S >= 1 && S <= kMaxSub, "sub-rounds must be in [1, 8]");
     KH_REQUIRE(bins.hi[kNumBins - 2] <= 512, "the wave bin holds degrees <= 512");
     subrounds_ = S;
@@ -870,6 +879,8 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     next_active_.reserve(g.n);
     big_.reserve(g.n);
     n_big_.reserve(kMaxSub * kMaxRounds);
+    n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
+    trav_.reserve(64);
 
     fill_async(counts_.p, L, 0, s);
     fill_async(cursor_.p, L, 0, s);
@@ -881,7 +892,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     fill_async(dtop_.p, num_labels, 0ull, s);
     if (g.n > 0)
         hipLaunchKernelGGL(bin_count, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
-                           seed, S, bins, counts_.p, edges_.p);
+                           seed, S, bins, include, counts_.p, edges_.p);
     KH_CHECK(hipMemcpyAsync(host_counts, counts_.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
     KH_CHECK(hipMemcpyAsync(host_edges, edges_.p, sizeof(long long) * L, hipMemcpyDeviceToHost,
                             s));
@@ -894,7 +905,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     KH_CHECK(hipMemcpyAsync(offsets_.p, host_offsets, sizeof(int) * L, hipMemcpyHostToDevice, s));
     if (g.n > 0)
         hipLaunchKernelGGL(bin_scatter, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
-                           seed, S, bins, offsets_.p, cursor_.p, lists_.p);
+                           seed, S, bins, include, offsets_.p, cursor_.p, lists_.p);
 
     // hubs (last bin): chunk prefix per sub-round list, global tables for multi-chunk hubs
     int hubs = 0;
@@ -1023,16 +1034,16 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
 static const char* kBinName[kNumBins] = {"g8", "g16", "g32", "g64", "wave", "hub"};
 
 void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int bound, int rounds,
-                      uint32_t seed, hipStream_t s, Profiler* prof, const char* phase) {
+                      uint32_t seed, hipStream_t s, Profiler* prof, const char* phase,
+                      const int* lbound) {
     KH_REQUIRE(rounds >= 0 && rounds <= kMaxRounds, "LP rounds must be in [0, 64]");
     const int S = subrounds_;
     const bool cl = mode == LpMode::Cluster;
     KH_REQUIRE(cl || !(dense_wave_ || dense_hub_) || num_labels_ > 0, "prepare() not called");
+    // no allocation here: run() may be captured into a hipGraph (prepare() sized everything)
     fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
-    n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     fill_async(n_big_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
-    trav_.reserve(64);
     fill_async(trav_.p, 64, 0ull, s);
     if (!cl) {
         fill_async(active_.p, g.n, (uint8_t)1, s);
@@ -1046,6 +1057,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.label = label;
     a.size = size;
     a.bound = bound;
+    a.lbound = lbound;
     a.inc = inc_.p;
     a.top = top_.p;
     a.out = out_.p;
@@ -1110,6 +1122,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         const int* ctb = ct_base_.p + hub_base_[sr];
                         if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        ps.close();  // the hub scope times propose_hub_chunk alone
                         if (hub_multi_n_[sr] > 0) {
                             ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);
                             const int* ml = multi_.p + multi_base_[sr];
@@ -1181,7 +1194,7 @@ void LpWorkspace::init_cluster_labels(const DGraph& g, int* label, int* size, hi
 }
 
 void compute_block_weights(const DGraph& g, const int* part, int k, int* W, hipStream_t s) {
-    KH_REQUIRE(k >= 1 && k <= 65536, "k must be in [1, 65536]");
+    KH_REQUIRE(k >= 1, "k must be >= 1");
     fill_async(W, k, 0, s);
     if (g.n == 0) return;
     if (k > 8192) {  // per-workgroup LDS histograms only up to 32 KiB

@@ -18,6 +18,7 @@ struct LpArgs {
     int* label;        // label of each node (cluster id or block id)
     int* size;         // weight of each label
     int bound;         // admission bound: size + w < bound
+    const int* lbound; // per-label bounds (a disjoint union of graphs), nullptr: `bound` for all
     uint32_t tie_seed;
     const uint8_t* active;  // refinement frontier (nullptr for clustering)
     uint8_t* next_active;
@@ -42,12 +43,15 @@ class LpWorkspace {
 public:
     // Bins the nodes of g into per-(sub-round, degree bin) lists and sizes the scratch for
     // num_labels labels. One small host readback.
+    // Nodes with include[v] == 0 (include may be null) are left out: they never propose.
     void prepare(LpMode mode, const DGraph& g, int num_labels, int subrounds, uint32_t seed,
-                 const BinConfig& bins, hipStream_t s);
+                 const BinConfig& bins, hipStream_t s, const uint8_t* include = nullptr);
     // Runs `rounds` synchronous rounds. label/size are device arrays (node -> label,
     // label -> weight). Refinement starts from an all-active frontier.
+    // lbound (optional): the bound of each label, replacing `bound`.
     void run(LpMode mode, const DGraph& g, int* label, int* size, int bound, int rounds,
-             uint32_t seed, hipStream_t s, Profiler* prof, const char* phase);
+             uint32_t seed, hipStream_t s, Profiler* prof, const char* phase,
+             const int* lbound = nullptr);
     // label[v] = v, size[v] = vwgt[v]
     static void init_cluster_labels(const DGraph& g, int* label, int* size, hipStream_t s);
     // Moves per round of the last run (synchronises the stream).

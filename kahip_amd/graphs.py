@@ -203,6 +203,27 @@ def gen_rgg(n, seed, radius=None):
     return xadj, adjncy, None, None
 
 
+def batch_spec(seed, i, log2_min=10.0, log2_max=16.0):
+    """Graph i of a generated batch: (kind, size, graph_seed) with kind 'rgg' (n = size) for
+    even i and 'rmat' (scale = size, 16 samples per node) for odd i;
+    x = log2_min + u (log2_max - log2_min), u = (d >> 11) 2^-53, d = rng_u64(seed, 3, i);
+    RGG n = floor(2^x + 0.5), R-MAT scale = floor(x + 0.5); graph seed = rng_u64(seed, 4, i)."""
+    d = int(rng_u64(seed, 3, i))
+    u = float(d >> 11) * (1.0 / 9007199254740992.0)
+    x = log2_min + u * (log2_max - log2_min)
+    gseed = int(rng_u64(seed, 4, i))
+    if i % 2 == 0:
+        return "rgg", int(math.floor(2.0 ** x + 0.5)), gseed
+    return "rmat", int(math.floor(x + 0.5)), gseed
+
+
+def batch_graph(seed, i, log2_min=10.0, log2_max=16.0):
+    kind, size, gseed = batch_spec(seed, i, log2_min, log2_max)
+    if kind == "rgg":
+        return gen_rgg(size, gseed)
+    return gen_rmat(size, 16 << size, gseed)
+
+
 RMAT_ABC = (0.57, 0.19, 0.19)
 
 

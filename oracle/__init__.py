@@ -80,6 +80,12 @@ def lib():
         L.orc_lp_refine_sync.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_uint,
                                          C.c_int, C.c_int, C.c_uint32, _i32p, ip]
         L.orc_lp_refine_sync.restype = None
+        u8p = C.POINTER(C.c_uint8)
+        L.orc_lp_cluster_sync_ex.argtypes = [C.c_int, _i32p, _i32p, ip, ip, _i32p, u8p, C.c_int,
+                                             C.c_int, C.c_uint32, _i32p, ip]
+        L.orc_lp_refine_sync_ex.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, _i32p, u8p,
+                                            C.c_int, C.c_int, C.c_uint32, _i32p, ip]
+        L.orc_lp_refine_sync_ex.restype = None
         _lib = L
     return _lib
 
@@ -213,6 +219,39 @@ def lp_cluster_sync(xadj, adjncy, vwgt, adjwgt, bound, rounds, subrounds, seed):
                                   subrounds, seed & 0xFFFFFFFF, out,
                                   moves.ctypes.data_as(C.POINTER(C.c_int)))
     return out, c, moves
+
+
+def _u8(mask):
+    if mask is None:
+        return None
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    return m, m.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def lp_cluster_sync_ex(xadj, adjncy, vwgt, adjwgt, lbound, include, rounds, subrounds, seed):
+    """R2 clustering with the bound of label (node) c in lbound[c]; nodes with include[v] == 0
+    (include may be None) never propose."""
+    out = np.empty(_n(xadj), np.int32)
+    moves = np.zeros(max(rounds, 1), np.int32)
+    inc = _u8(include)
+    c = lib().orc_lp_cluster_sync_ex(_n(xadj), xadj, adjncy, _p(vwgt), _p(adjwgt),
+                                     np.ascontiguousarray(lbound, dtype=np.int32),
+                                     None if inc is None else inc[1], rounds, subrounds,
+                                     seed & 0xFFFFFFFF, out,
+                                     moves.ctypes.data_as(C.POINTER(C.c_int)))
+    return out, c, moves
+
+
+def lp_refine_sync_ex(xadj, adjncy, vwgt, adjwgt, k, lbound, include, rounds, subrounds, seed,
+                      part):
+    p = np.array(part, dtype=np.int32)
+    moves = np.zeros(max(rounds, 1), np.int32)
+    inc = _u8(include)
+    lib().orc_lp_refine_sync_ex(_n(xadj), xadj, adjncy, _p(vwgt), _p(adjwgt), k,
+                                np.ascontiguousarray(lbound, dtype=np.int32),
+                                None if inc is None else inc[1], rounds, subrounds,
+                                seed & 0xFFFFFFFF, p, moves.ctypes.data_as(C.POINTER(C.c_int)))
+    return p, moves
 
 
 def lp_refine_sync(xadj, adjncy, vwgt, adjwgt, k, bound, rounds, subrounds, seed, part):

@@ -348,8 +348,10 @@ int best_label(int v, const int* xadj, const int* adjncy, const int* adjwgt, con
     return best;
 }
 
-// The admission rule above, applied to every target group of P; `size` is the snapshot.
-void admit(const std::vector<Proposal>& P, const std::vector<int64_t>& size, int64_t bound,
+// The admission rule above, applied to every target group of P; `size` is the snapshot and
+// bound(t) the bound of label t.
+template <class Bound>
+void admit(const std::vector<Proposal>& P, const std::vector<int64_t>& size, Bound bound,
            std::vector<Proposal>& out, std::vector<int64_t>& inc, std::vector<int>& top) {
     for (size_t i = 0; i < P.size(); ++i) {
         const Proposal& p = P[i];
@@ -358,7 +360,8 @@ void admit(const std::vector<Proposal>& P, const std::vector<int64_t>& size, int
     }
     for (size_t i = 0; i < P.size(); ++i) {
         const Proposal& p = P[i];
-        if (size[p.to] + inc[p.to] < bound || (top[p.to] == (int)i && size[p.to] + p.w < bound))
+        const int64_t b = bound(p.to);
+        if (size[p.to] + inc[p.to] < b || (top[p.to] == (int)i && size[p.to] + p.w < b))
             out.push_back(p);
     }
     for (const Proposal& p : P) {
@@ -378,9 +381,14 @@ extern "C" {
 // conn(c) > conn(own), or conn(c) == conn(own) and size(c) >= size(own). Admitted moves are
 // applied together; labels are then renumbered in first-appearance order. Returns the number of
 // clusters.
-int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
-                        const int* adjwgt, unsigned bound, int rounds, int subrounds,
-                        uint32_t seed, int* cluster_out, int* moves_per_round) {
+//
+// The _ex form takes the bound per label (lbound[c], labels being node ids; used by the batched
+// mode, where each graph of a disjoint union has its own bound) and an optional inclusion mask:
+// nodes with include[v] == 0 never propose (they keep their own label).
+int orc_lp_cluster_sync_ex(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                           const int* adjwgt, const int* lbound, const uint8_t* include,
+                           int rounds, int subrounds, uint32_t seed, int* cluster_out,
+                           int* moves_per_round) {
     std::vector<int> label(n), sub(n), top(n, -1);
     std::vector<int64_t> size(n), acc(n, 0), inc(n, 0);
     std::vector<int> touched;
@@ -390,7 +398,7 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
         sub[v] = subround_of(seed, v, subrounds);
     }
     std::vector<Proposal> P, A;
-    const int64_t B = (int64_t)bound;
+    auto bound = [&](int c) -> int64_t { return (int64_t)lbound[c]; };
     for (int r = 0; r < rounds; ++r) {
         int moves = 0;
         uint32_t tie_seed = seed ^ kTieSalt ^ ((uint32_t)r * kGolden);
@@ -398,18 +406,18 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
             P.clear();
             A.clear();
             for (int v = 0; v < n; ++v) {
-                if (sub[v] != s) continue;
+                if (sub[v] != s || (include && !include[v])) continue;
                 int own = label[v];
                 int w = wv(vwgt, v);
                 int64_t conn_own, conn_best;
                 int best = best_label(v, xadj, adjncy, adjwgt, label.data(), own, tie_seed, acc,
                                       touched, &conn_own, &conn_best,
-                                      [&](int c) { return size[c] + w < B; });
+                                      [&](int c) { return size[c] + w < bound(c); });
                 if (best >= 0 && (conn_best > conn_own ||
                                   (conn_best == conn_own && size[best] >= size[own])))
                     P.push_back({v, own, best, conn_best - conn_own, w});
             }
-            admit(P, size, B, A, inc, top);
+            admit(P, size, bound, A, inc, top);
             for (const Proposal& p : A) {
                 size[p.from] -= p.w;
                 size[p.to] += p.w;
@@ -423,6 +431,14 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
     return orc_remap_first_appearance(n, cluster_out);
 }
 
+int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                        const int* adjwgt, unsigned bound, int rounds, int subrounds,
+                        uint32_t seed, int* cluster_out, int* moves_per_round) {
+    std::vector<int> lb(n, (int)std::min<unsigned>(bound, 0x7fffffffu));
+    return orc_lp_cluster_sync_ex(n, xadj, adjncy, vwgt, adjwgt, lb.data(), nullptr, rounds,
+                                  subrounds, seed, cluster_out, moves_per_round);
+}
+
 // Synchronous LP refinement (R2). Block weights W. Round 0 has every node active; later rounds
 // the neighbours of the nodes moved in the previous round (label_propagation_refinement.cpp
 // :84-92). In sub-round s an active node v with subround(v) == s proposes its best neighbouring
@@ -431,9 +447,12 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
 // own ineligible at :68). Departures from an overloaded block b all pass when W(b) minus their
 // summed weight stays >= bound; otherwise only the highest-priority departure passes. The
 // surviving proposals are admitted as in the clustering and applied together.
-void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
-                        const int* adjwgt, int k, unsigned bound, int rounds, int subrounds,
-                        uint32_t seed, int* part, int* moves_per_round) {
+//
+// The _ex form takes the bound per block (lbound[b], b < k) and an inclusion mask as above.
+void orc_lp_refine_sync_ex(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                           const int* adjwgt, int k, const int* lbound, const uint8_t* include,
+                           int rounds, int subrounds, uint32_t seed, int* part,
+                           int* moves_per_round) {
     std::vector<int64_t> W(k, 0), acc(k, 0), inc(k, 0), out(k, 0);
     std::vector<int> touched, top(k, -1), dtop(k, -1), sub(n);
     for (int v = 0; v < n; ++v) {
@@ -442,7 +461,7 @@ void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vw
     }
     std::vector<char> active(n, 1), next(n, 0);
     std::vector<Proposal> P, D, A;
-    const int64_t U = (int64_t)bound;
+    auto U = [&](int b) -> int64_t { return (int64_t)lbound[b]; };
     for (int r = 0; r < rounds; ++r) {
         int moves = 0;
         uint32_t tie_seed = seed ^ kTieSalt ^ ((uint32_t)r * kGolden);
@@ -452,27 +471,28 @@ void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vw
             D.clear();
             A.clear();
             for (int v = 0; v < n; ++v) {
-                if (!active[v] || sub[v] != s) continue;
+                if (!active[v] || sub[v] != s || (include && !include[v])) continue;
                 int own = part[v];
                 int w = wv(vwgt, v);
                 int64_t conn_own, conn_best;
                 int best = best_label(v, xadj, adjncy, adjwgt, part, own, tie_seed, acc, touched,
                                       &conn_own, &conn_best,
-                                      [&](int b) { return W[b] + w < U; });
-                bool overloaded = W[own] > U;
+                                      [&](int b) { return W[b] + w < U(b); });
+                bool overloaded = W[own] > U(own);
                 if (best >= 0 && (conn_best > conn_own || overloaded ||
                                   (conn_best == conn_own && W[best] + 2 * (int64_t)w <= W[own])))
                     P.push_back({v, own, best, conn_best - conn_own, w});
             }
             for (size_t i = 0; i < P.size(); ++i) {
                 const Proposal& p = P[i];
-                if (W[p.from] <= U) continue;
+                if (W[p.from] <= U(p.from)) continue;
                 out[p.from] += p.w;
                 if (dtop[p.from] < 0 || higher_priority(p, P[dtop[p.from]])) dtop[p.from] = (int)i;
             }
             for (size_t i = 0; i < P.size(); ++i) {
                 const Proposal& p = P[i];
-                if (W[p.from] <= U || W[p.from] - out[p.from] >= U || dtop[p.from] == (int)i)
+                if (W[p.from] <= U(p.from) || W[p.from] - out[p.from] >= U(p.from) ||
+                    dtop[p.from] == (int)i)
                     D.push_back(p);
             }
             for (const Proposal& p : P) {
@@ -491,6 +511,14 @@ void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vw
         if (moves_per_round) moves_per_round[r] = moves;
         std::swap(active, next);
     }
+}
+
+void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                        const int* adjwgt, int k, unsigned bound, int rounds, int subrounds,
+                        uint32_t seed, int* part, int* moves_per_round) {
+    std::vector<int> lb(k, (int)std::min<unsigned>(bound, 0x7fffffffu));
+    orc_lp_refine_sync_ex(n, xadj, adjncy, vwgt, adjwgt, k, lb.data(), nullptr, rounds,
+                          subrounds, seed, part, moves_per_round);
 }
 
 }  // extern "C"

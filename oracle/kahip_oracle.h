@@ -53,6 +53,15 @@ int orc_lp_cluster_sync(int n, const int* xadj, const int* adjncy, const int* vw
 void orc_lp_refine_sync(int n, const int* xadj, const int* adjncy, const int* vwgt,
                         const int* adjwgt, int k, unsigned bound, int rounds, int subrounds,
                         uint32_t seed, int* part, int* moves_per_round);
+/* Per-label bounds and an optional inclusion mask (batched mode over a disjoint union). */
+int orc_lp_cluster_sync_ex(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                           const int* adjwgt, const int* lbound, const uint8_t* include,
+                           int rounds, int subrounds, uint32_t seed, int* cluster_out,
+                           int* moves_per_round);
+void orc_lp_refine_sync_ex(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                           const int* adjwgt, int k, const int* lbound, const uint8_t* include,
+                           int rounds, int subrounds, uint32_t seed, int* part,
+                           int* moves_per_round);
 
 #ifdef __cplusplus
 }

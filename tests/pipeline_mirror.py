@@ -70,3 +70,101 @@ def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, clus
             "level_n": [len(lv[0]) - 1 for lv in levels], "initial": initial,
             "coarsest": levels[level]}
     return part, info
+
+
+def mirror_batch(graphs, k, imbalance=0.03, seed=0, cluster_rounds=10, refine_rounds=25,
+                 subrounds=4, cluster_factor=18, stop_factor=5000, ip=None):
+    """CPU mirror of the batched mode: the disjoint union of `graphs` ((xadj, adjncy, vwgt,
+    adjwgt) with local ids) under the synchronous schedule with per-label bounds (the cluster
+    bound of a node's graph; the block bound U_g of labels g*k + b), per-graph stop rules and
+    per-graph coarsest levels. Returns (local parts per graph, info)."""
+    if ip is None:
+        import kahip_amd
+        ip = kahip_amd.initial_partition
+    G = len(graphs)
+    # the union
+    noff = [0]
+    xs, adjs, vws, ews = [], [], [], []
+    eoff = 0
+    for x, a, vw, ew in graphs:
+        n = len(x) - 1
+        xs.append(np.asarray(x[:-1], np.int64) + eoff)
+        adjs.append(np.asarray(a, np.int64) + noff[-1])
+        vws.append(np.ones(n, np.int64) if vw is None else np.asarray(vw, np.int64))
+        ews.append(np.ones(int(x[-1]), np.int64) if ew is None else np.asarray(ew, np.int64))
+        eoff += int(x[-1])
+        noff.append(noff[-1] + n)
+    ux = np.concatenate(xs + [np.array([eoff])]).astype(np.int32)
+    ua = np.concatenate(adjs).astype(np.int32) if eoff else np.zeros(0, np.int32)
+    uv = np.concatenate(vws).astype(np.int32)
+    uw = np.concatenate(ews).astype(np.int32) if eoff else np.zeros(0, np.int32)
+    W = [int(v.sum()) for v in vws]
+    U = [O.upper_bound(w, k, imbalance) for w in W]
+    B = [O.cluster_bound(u, cluster_factor) for u in U]
+    levels = [(ux, ua, uv, uw)]
+    offs = [np.array(noff, np.int64)]
+    maps = []
+    on = np.ones(G, bool)
+    coarsest = np.zeros(G, np.int64)
+    level = 0
+    while on.any():
+        x, a, vw, ew = levels[level]
+        n = len(x) - 1
+        o = offs[level]
+        ng = np.repeat(np.arange(G), np.diff(o))
+        nbound = np.array(B, np.int64)[ng].astype(np.int32)
+        include = on[ng].astype(np.uint8)
+        cs = O.hash3(seed, 0xC1, level)
+        cl, nc, _ = O.lp_cluster_sync_ex(x, a, vw, ew, nbound, include, cluster_rounds,
+                                         subrounds, cs)
+        cx, ca, cw, cv = O.contract(x, a, vw, ew, cl, nc)
+        maps.append(cl)
+        levels.append((cx, ca, cv, cw))
+        co = np.array([cl[o[g]] for g in range(G)] + [nc], np.int64)
+        offs.append(co)
+        level += 1
+        for g in range(G):
+            if on[g] and not O.stop_rule_continue(int(o[g + 1] - o[g]), int(co[g + 1] - co[g]),
+                                                  k, stop_factor):
+                on[g] = False
+                coarsest[g] = level
+    # initial partitions per graph at its coarsest level
+    reps = ip_repetitions(k)
+    base = ip_seed(seed)
+    ip_part = {}
+    for g in range(G):
+        l = int(coarsest[g])
+        x, a, vw, ew = levels[l]
+        o = offs[l]
+        lo, hi = int(o[g]), int(o[g + 1])
+        e0, e1 = int(x[lo]), int(x[hi])
+        sx = (np.asarray(x[lo:hi + 1], np.int64) - e0).astype(np.int32)
+        sa = (np.asarray(a[e0:e1], np.int64) - lo).astype(np.int32)
+        sv = np.asarray(vw[lo:hi], np.int32)
+        sw = np.asarray(ew[e0:e1], np.int32)
+        gseed = base ^ ((O.hash3(seed, 0xB7, g) << 1) & 0xFFFFFFFFFFFFFFFF)
+        ip_part[g] = np.asarray(ip(sx, sa, sv, sw, k, U[g], reps, gseed), np.int64)
+    # uncoarsening
+    lbound = np.repeat(np.array(U, np.int64), k).astype(np.int32)
+    part = None
+    for l in range(level, -1, -1):
+        x, a, vw, ew = levels[l]
+        o = offs[l]
+        ng = np.repeat(np.arange(G), np.diff(o))
+        if l == level:
+            part = (ng * k).astype(np.int32)
+        else:
+            part = O.project(maps[l], part)
+        for g in range(G):
+            if coarsest[g] == l and l > 0:
+                part[int(o[g]):int(o[g + 1])] = g * k + ip_part[g]
+        include = (coarsest[ng] >= l).astype(np.uint8)
+        rs = O.hash3(seed, 0x4E, l)
+        part, _ = O.lp_refine_sync_ex(x, a, vw, ew, G * k, lbound, include, refine_rounds,
+                                      subrounds, rs, part)
+    out = []
+    for g in range(G):
+        out.append(part[noff[g]:noff[g + 1]] - g * k)
+    info = {"U": U, "B": B, "levels": level, "coarsest": coarsest.tolist(),
+            "union": levels[0], "node_off": noff}
+    return out, info

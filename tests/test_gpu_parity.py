@@ -190,6 +190,17 @@ def test_rmat_generator_matches_numpy(scale, samples, seed):
     g.free()
 
 
+@pytest.mark.parametrize("n,seed", [(2, 0), (1000, 1), (32768, 2), (100000, 3)])
+def test_rgg_generator_matches_numpy(n, seed):
+    g = K.Graph.rgg(n, seed)
+    x, a, vw, ew = g.to_host()
+    wx, wa, _, _ = K.gen_rgg(n, seed)
+    assert np.array_equal(x, wx) and np.array_equal(a, wa)
+    assert np.all(vw == 1) and np.all(ew == 1)
+    assert g.info()["max_degree"] == int(np.diff(wx).max())
+    g.free()
+
+
 def test_device_graph_partition_roundtrip(G):
     x, a, vw, ew = G["rgg15"]
     g = K.Graph.from_host(x, a, vw, ew)
