@@ -132,10 +132,12 @@ def test_learned_scores_are_quantised_logits(graphs, name):
     f = K.edge_features(x, a, vw, ew, 40, 99)
     z = K.mlp_forward(f, params)
     assert np.array_equal(s, R.quantize_logit(z, R.score_shift(int(np.diff(x).max()))))
-    # against the oracle's float64 chain: equal but for logits next to a rounding boundary
-    want = R.quantize_logit(R.mlp_forward(R.edge_features(x, a, vw, ew, 40, 99), params))
+    # against the oracle's float64 chain with the same BF16 roundings: equal but for logits
+    # next to a rounding boundary of the 1/128 score grid
+    want = R.quantize_logit(R.mlp_forward(R.edge_features(x, a, vw, ew, 40, 99), params,
+                                          bf16=True))
     assert np.mean(s == want) >= 0.97
-    assert np.max(np.abs(s.astype(np.int64) - want)) <= 8
+    assert np.max(np.abs(s.astype(np.int64) - want)) <= 2
 
 
 @pytest.mark.parametrize("name,k,seed", [("rgg15", 2, 0), ("rmat12", 8, 1), ("weighted132", 2, 2)])
