@@ -22,7 +22,6 @@ import time
 import numpy as np
 
 import kahip_amd as K
-from kahip_amd import graphs as G
 
 FEATURES, HIDDEN = K.MLP_FEATURES, K.MLP_HIDDEN
 OFF_W1 = 0
@@ -49,8 +48,7 @@ def training_graph(i):
     rng = np.random.default_rng(seed)
     lg = int(rng.integers(16, 21))
     if i % 2 == 0:
-        x, a, _, _ = G.gen_rgg(1 << lg, seed)
-        return "rgg", lg, K.Graph.from_host(x, a)
+        return "rgg", lg, K.Graph.rgg(1 << lg, seed)
     return "rmat", lg, K.Graph.rmat(lg, 16 << lg, seed)
 
 
