@@ -39,7 +39,74 @@ WORKLOADS = {
     # smaller variants for quick checks
     "rmat20": dict(kind="rmat", scale=20, samples=1 << 24, k=256, seed=1),
     "rmat16": dict(kind="rmat", scale=16, samples=1 << 20, k=64, seed=1),
+    # random geometric graph, n = 2^22, r = 0.55 sqrt(ln n / n), k = 64, eps = 3 %
+    "rgg22": dict(kind="rgg", n=1 << 22, k=64, seed=1),
 }
+
+
+BATCH = dict(graphs=4096, log2_min=10.0, log2_max=16.0, k=8, seed=1, micro=64)
+
+
+def run_batch(K, args):
+    """Batched workload: 4096 graphs (RGG / R-MAT alternating, n log-uniform in [2^10, 2^16],
+    k = 8, eps = 3 %) in one disjoint union; graphs/s of whole-batch calls, plus per-graph
+    latency when the same graphs are submitted as micro-batches of BATCH['micro'] graphs."""
+    import numpy as np
+    w = BATCH
+    t0 = time.time()
+    b = K.Batch.generate(w["graphs"], w["log2_min"], w["log2_max"], seed=w["seed"])
+    inf = b.info()
+    log(f"batch: {inf['graphs']} graphs, n={inf['n']} 2m={inf['m2']} generated in "
+        f"{time.time() - t0:.1f}s")
+    bins, tune_ms = K.autotune_bins(b, k=w["k"])
+    log(f"autotuned bins {bins} (ms per candidate {[round(t, 2) for t in tune_ms]})")
+    for i in range(args.warmup):
+        b.partition(k=w["k"], seed=0, bins=bins, use_graph=True)
+    walls, stats = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        _, cuts, mbw, st = b.partition(k=w["k"], seed=0, bins=bins, use_graph=True)
+        walls.append(time.perf_counter() - t)
+        stats.append(st)
+        log(f"step {i}: wall {walls[-1]*1e3:.1f} ms gpu {st['ms_coarsen'] + st['ms_refine'] + st['ms_cut']:.1f} ms "
+            f"ip {st['ms_initial']:.1f} ms")
+    # latency: micro-batches of consecutive graphs, each its own call
+    x, a, vw, ew = b.to_host()
+    off = inf["node_off"]
+    lat = []
+    for lo in range(0, w["graphs"], w["micro"]):
+        hi = min(lo + w["micro"], w["graphs"])
+        gs = []
+        for g in range(lo, hi):
+            e0, e1 = x[off[g]], x[off[g + 1]]
+            gs.append((x[off[g]:off[g + 1] + 1] - e0, a[e0:e1] - off[g], None, ew[e0:e1]))
+        mb = K.Batch.from_graphs(gs)
+        t = time.perf_counter()
+        mb.partition(k=w["k"], seed=0, bins=bins, use_graph=True)
+        lat += [time.perf_counter() - t] * (hi - lo)
+        mb.free()
+    b.free()
+    lat = np.array(lat) * 1e3
+    gps = w["graphs"] / (sum(walls) / len(walls))
+    gpu = [s["ms_coarsen"] + s["ms_refine"] + s["ms_cut"] for s in stats]
+    edges = [s["edges_cluster"] + s["edges_contract"] + s["edges_refine"] + s["edges_cut"]
+             for s in stats]
+    out = {"metric": "batched LP partitioning throughput (graphs/s)", "value": round(gps, 2),
+           "unit": "graphs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(sum(walls) / len(walls) * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": (f"{w['graphs']} graphs, RGG/R-MAT alternating, n log-uniform in "
+                                   f"[2^{w['log2_min']:.0f}, 2^{w['log2_max']:.0f}], k={w['k']} "
+                                   f"eps=0.03, one disjoint union, hipGraph LP loops"),
+                      "n": inf["n"], "m2": inf["m2"], "bins": bins},
+           "gpu_ms_per_step": round(sum(gpu) / len(gpu), 2),
+           "gpu_gteps": round(sum(edges) / (sum(gpu) / 1e3) / 1e9, 3),
+           "cpu_initial_partition_ms_per_step": round(
+               sum(s["ms_initial"] for s in stats) / len(stats), 1),
+           "latency_ms_micro_batches": {"micro_batch": w["micro"],
+                                        "p50": round(float(np.percentile(lat, 50)), 2),
+                                        "p99": round(float(np.percentile(lat, 99)), 2)}}
+    print(json.dumps(out), flush=True)
 
 
 def log(msg):
@@ -49,6 +116,8 @@ def log(msg):
 def make_graph(K, w):
     if w["kind"] == "rmat":
         return K.Graph.rmat(w["scale"], w["samples"], w["seed"])
+    if w["kind"] == "rgg":
+        return K.Graph.rgg(w["n"], w["seed"])
     raise ValueError(w["kind"])
 
 
@@ -111,7 +180,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch4096"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="one profiled step, no timing loop (for rocprofv3 runs)")
@@ -124,6 +193,9 @@ def main():
 
     import kahip_amd as K
     K.lib()
+    if args.workload == "batch4096":
+        run_batch(K, args)
+        return
     w = WORKLOADS[args.workload]
     t0 = time.time()
     g = make_graph(K, w)
@@ -208,9 +280,11 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": (f"R-MAT scale {w.get('scale')} with {w.get('samples')} samples "
-                                f"(0.57,0.19,0.19,0.05), symmetrised, dedup weight=multiplicity; "
-                                f"kaffpa FASTSOCIAL k={w['k']} eps=0.03"),
+        "config": {"workload": ((f"R-MAT scale {w.get('scale')} with {w.get('samples')} samples "
+                                 f"(0.57,0.19,0.19,0.05), symmetrised, dedup weight=multiplicity; ")
+                                if w["kind"] == "rmat" else
+                                (f"RGG n={w.get('n')} r=0.55*sqrt(ln n/n), unit weights; ")) +
+                                f"kaffpa FASTSOCIAL k={w['k']} eps=0.03",
                    "n": info["n"], "m2": info["m2"], "k": w["k"], "imbalance": 0.03,
                    "parallelism": "single-gpu"},
         "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),

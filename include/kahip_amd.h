@@ -32,6 +32,9 @@ typedef struct {
     int rating;          /* edge rating of the LP clustering: KAHIP_RATING_* */
     int rating_levels;   /* the rating applies to coarsening levels < rating_levels */
     const float* mlp_weights; /* KAHIP_MLP_PARAMS floats (host), for KAHIP_RATING_LEARNED */
+    int reorder;         /* renumber the input by degree first: -1 automatic (maximum degree
+                            above 32 x the average), 0 off, 1 on; the partition is returned in
+                            the caller's numbering */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);
@@ -62,6 +65,7 @@ typedef struct {
     long long edges_cluster, edges_contract, edges_refine, edges_cut;
     long long max_block_weight;
     double ms_rating; /* rating time, part of ms_coarsen */
+    int reordered;    /* the input was renumbered by degree */
 } kahip_stats;
 
 void kahip_default_params(kahip_params* p);

@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("cluster_factor", C.c_int), ("stop_factor", C.c_int),
                 ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
                 ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
-                ("mlp_weights", C.POINTER(C.c_float))]
+                ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -68,7 +68,8 @@ class Stats(C.Structure):
                 ("ms_refine", C.c_double), ("ms_cut", C.c_double),
                 ("edges_cluster", C.c_longlong), ("edges_contract", C.c_longlong),
                 ("edges_refine", C.c_longlong), ("edges_cut", C.c_longlong),
-                ("max_block_weight", C.c_longlong), ("ms_rating", C.c_double)]
+                ("max_block_weight", C.c_longlong), ("ms_rating", C.c_double),
+                ("reordered", C.c_int)]
 
     def as_dict(self):
         L = self.levels
@@ -82,6 +83,7 @@ class Stats(C.Structure):
             "edges_cluster": int(self.edges_cluster), "edges_contract": int(self.edges_contract),
             "edges_refine": int(self.edges_refine), "edges_cut": int(self.edges_cut),
             "max_block_weight": int(self.max_block_weight), "ms_rating": self.ms_rating,
+            "reordered": bool(self.reordered),
         }
 
 

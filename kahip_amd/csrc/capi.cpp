@@ -68,6 +68,7 @@ PartitionParams to_params(const kahip_params* q) {
     p.rating = (RatingMode)q->rating;
     p.rating_levels = q->rating_levels;
     p.mlp = reinterpret_cast<const MlpWeights*>(q->mlp_weights);
+    p.reorder = q->reorder;
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
@@ -129,6 +130,7 @@ void fill_stats(const PartitionStats& s, kahip_stats* o) {
     o->edges_cut = s.edges_cut;
     o->max_block_weight = s.max_block_weight;
     o->ms_rating = s.ms_rating;
+    o->reordered = s.reordered ? 1 : 0;
 }
 
 // Greedy CPU rebalancing for kaffpa_balance(perfectly_balance): while a block exceeds `bound`,
@@ -236,6 +238,7 @@ void kahip_default_params(kahip_params* p) {
     p->rating = (int)d.rating;
     p->rating_levels = d.rating_levels;
     p->mlp_weights = nullptr;
+    p->reorder = d.reorder;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }

@@ -34,6 +34,7 @@ namespace {
 
 constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
 constexpr int kRowMaxDistinct = 512;
+constexpr int kRowSmall = 128;    // rows with summed member degree <= this use 256 slots
 constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
 constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
 constexpr int kBigMaxDistinct = 4096;
@@ -81,24 +82,59 @@ __global__ void scatter_members(int n, const int* __restrict__ map, const int* _
 }
 
 // One wave (= one workgroup) per coarse node.
+// Rows of coarse nodes by summed member degree: fdeg <= kRowSmall go to the small-LDS variant
+// (it cannot overflow), the rest to the kRowCap variant. Workgroup-aggregated list appends.
+__global__ __launch_bounds__(256) void row_bins(int nc, const int* __restrict__ fdeg,
+                                                int* __restrict__ small, int* __restrict__ mid,
+                                                int* __restrict__ counts) {
+    __shared__ int ls, lm, bs, bm;
+    if (threadIdx.x == 0) ls = lm = 0;
+    __syncthreads();
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int cls = -1, rank = 0;
+    if (p < nc) {
+        cls = fdeg[p] <= kRowSmall ? 0 : 1;
+        rank = atomicAdd(cls == 0 ? &ls : &lm, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bs = ls ? atomicAdd(&counts[0], ls) : 0;
+        bm = lm ? atomicAdd(&counts[1], lm) : 0;
+    }
+    __syncthreads();
+    if (cls == 0) small[bs + rank] = p;
+    else if (cls == 1) mid[bm + rank] = p;
+}
+
+// One wave per coarse row. The members' edges are expanded with a load-balanced scan (64
+// members at a time, each lane takes every 64th edge of the batch), their targets' clusters are
+// gathered kEpl at a time, and (q -> sum w, min e) is aggregated in an LDS hash; then the
+// entries are sorted by min e (LDS bitonic sort) and written at the row's capacity offset. With
+// CHECK, more than CAP / 2 distinct entries abandon the row to the overflow list.
+template <int CAP, bool CHECK>
 __global__ __launch_bounds__(64) void rows_lds(
-    int nc, const int* __restrict__ xadj, const int* __restrict__ adjncy,
-    const int* __restrict__ adjwgt, const int* __restrict__ map, const int* __restrict__ moff,
-    const int* __restrict__ members, const int* __restrict__ fdeg, const int* __restrict__ tpos,
-    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw,
-    int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
-    __shared__ int K[kRowCap];
-    __shared__ int V[kRowCap];
-    __shared__ int M[kRowCap];
-    __shared__ unsigned long long S[kRowMaxDistinct];
+    const int* __restrict__ rows, int nrows, const int* __restrict__ xadj,
+    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
+    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
+    const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
+    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    constexpr int MAXD = CAP / 2;
+    constexpr int kEpl = 4;
+    __shared__ int K[CAP];
+    __shared__ int V[CAP];
+    __shared__ int M[CAP];
+    __shared__ unsigned long long S[MAXD];
+    __shared__ int mpre[64];
+    __shared__ int mstart[64];
     __shared__ int used;
     __shared__ int over;
-    const int p = blockIdx.x;
+    __shared__ int nkeys;
+    if ((int)blockIdx.x >= nrows) return;  // uniform
+    const int p = rows[blockIdx.x];
     const int lane = threadIdx.x;
-    if (p >= nc) return;  // uniform
     const int fd = fdeg[p];
     int cap = 64;
-    while (cap < 2 * fd && cap < kRowCap) cap <<= 1;
+    while (cap < 2 * fd && cap < CAP) cap <<= 1;
     for (int i = lane; i < cap; i += 64) {
         K[i] = -1;
         V[i] = 0;
@@ -107,26 +143,65 @@ __global__ __launch_bounds__(64) void rows_lds(
     if (lane == 0) {
         used = 0;
         over = 0;
+        nkeys = 0;
     }
     __syncthreads();
     const int mb = moff[p], me = moff[p + 1];
     for (int base = mb; base < me; base += 64) {
         const int mi = base + lane;
-        if (mi < me && !over) {
+        int b = 0, d = 0;
+        if (mi < me) {
             const int u = members[mi];
-            for (int e = xadj[u]; e < xadj[u + 1]; ++e) {
-                const int q = map[adjncy[e]];
-                if (q == p) continue;
-                if (over) break;
+            b = xadj[u];
+            d = xadj[u + 1] - b;
+        }
+        int incl = d;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const int total = __shfl(incl, 63, 64);
+        mpre[lane] = incl;
+        mstart[lane] = b;
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += 64 * kEpl) {
+            int es[kEpl], qs[kEpl], ws[kEpl];
+#pragma unroll
+            for (int j = 0; j < kEpl; ++j) {
+                const int t = t0 + lane + 64 * j;
+                es[j] = -1;
+                if (t < total) {
+                    int lo = 0, hi = 63;  // first member whose inclusive prefix exceeds t
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (mpre[mid] > t) hi = mid; else lo = mid + 1;
+                    }
+                    es[j] = mstart[lo] + (t - (lo ? mpre[lo - 1] : 0));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kEpl; ++j) {
+                qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
+                ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kEpl; ++j)
+                if (es[j] >= 0) qs[j] = map[qs[j]];
+#pragma unroll
+            for (int j = 0; j < kEpl; ++j) {
+                const int q = qs[j];
+                if (es[j] < 0 || q == p) continue;
+                if (CHECK && over) break;
                 uint32_t h = mix32((uint32_t)q) & (uint32_t)(cap - 1);
                 while (true) {
                     const int old = atomicCAS(&K[h], -1, q);
-                    if (old == -1) {
-                        if (atomicAdd(&used, 1) >= kRowMaxDistinct) over = 1;
+                    if (CHECK && old == -1) {
+                        if (atomicAdd(&used, 1) >= MAXD) over = 1;
                     }
                     if (old == -1 || old == q) {
-                        atomicAdd(&V[h], adjwgt[e]);
-                        atomicMin(&M[h], e);
+                        atomicAdd(&V[h], ws[j]);
+                        atomicMin(&M[h], es[j]);
                         break;
                     }
                     h = (h + 1) & (uint32_t)(cap - 1);
@@ -134,17 +209,13 @@ __global__ __launch_bounds__(64) void rows_lds(
             }
         }
         __syncthreads();
-        if (over) break;  // uniform after the barrier
+        if (CHECK && over) break;  // uniform after the barrier
     }
-    __syncthreads();
-    if (over) {
+    if (CHECK && over) {
         if (lane == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
         return;
     }
     // compact (min e, slot) keys, pad to a power of two, bitonic sort ascending
-    __shared__ int nkeys;
-    if (lane == 0) nkeys = 0;
-    __syncthreads();
     for (int i = lane; i < cap; i += 64) {
         if (K[i] >= 0) {
             const int idx = atomicAdd(&nkeys, 1);
@@ -567,10 +638,26 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
         const double bytes = 12.0 * (double)m2 + 12.0 * n;
+        rsmall_.reserve(nc);
+        rmid_.reserve(nc);
+        rcount_.reserve(2);
+        fill_async(rcount_.p, 2, 0, s);
+        hipLaunchKernelGGL(row_bins, dim3(grid_for(nc, 256)), dim3(256), 0, s, nc, fdeg_.p,
+                           rsmall_.p, rmid_.p, rcount_.p);
+        int rc[2] = {0, 0};
+        KH_CHECK(hipMemcpyAsync(rc, rcount_.p, sizeof(rc), hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
         ProfScope ps(prof, "contract_rows_lds", bytes, m2, s);
-        hipLaunchKernelGGL(rows_lds, dim3(nc), dim3(64), 0, s, nc, g.xadj, g.adjncy, g.adjwgt,
-                           map, moff_.p, members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p,
-                           ovl_list_.p, ovl_count_.p);
+        if (rc[0] > 0)
+            hipLaunchKernelGGL((rows_lds<2 * kRowSmall, false>), dim3(rc[0]), dim3(64), 0, s,
+                               rsmall_.p, rc[0], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
+                               members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
+                               ovl_count_.p);
+        if (rc[1] > 0)
+            hipLaunchKernelGGL((rows_lds<kRowCap, true>), dim3(rc[1]), dim3(64), 0, s, rmid_.p,
+                               rc[1], g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
+                               fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
+                               ovl_count_.p);
     }
     if (nc > 0) {
         ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);

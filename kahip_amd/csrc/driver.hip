@@ -10,6 +10,8 @@
 
 #include "driver.hpp"
 
+#include "reorder.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -114,12 +116,12 @@ Engine::~Engine() {
     if (s_) (void)hipStreamDestroy(s_);
 }
 
-void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out,
+void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_out,
                        PartitionStats* st) {
     KH_REQUIRE(p.k >= 1, "nparts must be >= 1");
     KH_REQUIRE(p.k <= 65536, "nparts must be <= 65536");
     KH_REQUIRE(p.imbalance >= 0, "imbalance must be >= 0");
-    KH_REQUIRE(g0.total_vwgt < (1LL << 31), "total node weight must be < 2^31");
+    KH_REQUIRE(input.total_vwgt < (1LL << 31), "total node weight must be < 2^31");
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs MLP weights");
     PartitionStats local;
     PartitionStats& S = st ? *st : local;
@@ -129,19 +131,27 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
     this->prof.enabled = p.profile;
     this->prof.reset();
 
-    const unsigned U = upper_bound_for(g0.total_vwgt, p.k, p.imbalance);
+    const unsigned U = upper_bound_for(input.total_vwgt, p.k, p.imbalance);
     const unsigned B = cluster_bound_for(U, p.cluster_factor);
     KH_REQUIRE(U < (1u << 31), "block bound exceeds int32");
     S.upper_bound = U;
     S.cluster_bound = B;
 
     // ------------------------------------------------------------------ coarsening
+    // Skewed graphs are renumbered by degree first (part of the coarsening time); the pipeline
+    // runs on the renumbered graph and the partition is mapped back at the end.
+    OwnedGraph reordered;
+    DevBuf<int> inv;
+    const bool reorder = reorder_wanted(input, p.reorder);
+    EventPair ev_coarsen;
+    KH_CHECK(hipEventRecord(ev_coarsen.a, s));
+    if (reorder) reorder_by_degree(input, reordered, inv, s, prof);
+    const DGraph& g0 = reorder ? reordered.g : input;
+    S.reordered = reorder;
     std::vector<std::unique_ptr<OwnedGraph>> coarse;  // coarse[l] = level l+1
     std::vector<DevBuf<int>> maps;                     // maps[l]: level l -> level l+1
     auto level_graph = [&](int l) -> const DGraph& { return l == 0 ? g0 : coarse[l - 1]->g; };
     DevBuf<int> label, size, score;
-    EventPair ev_coarsen;
-    KH_CHECK(hipEventRecord(ev_coarsen.a, s));
     const unsigned num_stop = (unsigned)p.stop_factor * (unsigned)p.k;
     int level = 0;
     S.level_n[0] = g0.n;
@@ -250,8 +260,15 @@ void Engine::partition(const DGraph& g0, const PartitionParams& p, int* part_out
     compute_block_weights(g0, part_c.p, p.k, W.p, s);
     std::vector<int> hw(p.k);
     KH_CHECK(hipMemcpyAsync(hw.data(), W.p, sizeof(int) * p.k, hipMemcpyDeviceToHost, s));
-    if (g0.n > 0)
-        KH_CHECK(hipMemcpyAsync(part_out, part_c.p, sizeof(int) * g0.n, hipMemcpyDeviceToHost, s));
+    if (g0.n > 0) {
+        const int* out_part = part_c.p;
+        if (reorder) {
+            part_f.reserve(g0.n);
+            unpermute_parts(g0.n, inv.p, part_c.p, part_f.p, s);
+            out_part = part_f.p;
+        }
+        KH_CHECK(hipMemcpyAsync(part_out, out_part, sizeof(int) * g0.n, hipMemcpyDeviceToHost, s));
+    }
     KH_CHECK(hipStreamSynchronize(s));
     S.ms_cut = ev_cut.ms();
     long long mx = 0;

@@ -34,6 +34,7 @@ struct PartitionParams {
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels
     // (None: the edge weights themselves, as in the FASTSOCIAL preset).
     RatingMode rating = RatingMode::None;
+    int reorder = -1;  // degree renumbering of the input: -1 automatic (skewed), 0 off, 1 on
     int rating_levels = kMaxLevels;
     const MlpWeights* mlp = nullptr;  // required for RatingMode::Learned
 };
@@ -48,6 +49,7 @@ struct PartitionStats {
     long long level_m2[kMaxLevels + 1] = {};
     double ms_coarsen = 0, ms_initial = 0, ms_refine = 0, ms_cut = 0;
     double ms_rating = 0;  // part of ms_coarsen
+    bool reordered = false;
     long long edges_cluster = 0, edges_contract = 0, edges_refine = 0, edges_cut = 0;
     long long max_block_weight = 0;
 };

@@ -30,6 +30,7 @@ namespace kahip {
 namespace {
 
 constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => load <= 1/2)
+static_assert(kWaveCap == 1024, "propose_wave loads up to kWaveCap / 2 edges per wave");
 constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
 constexpr int kBlock = 256;
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
@@ -40,12 +41,11 @@ constexpr int kHubDenseSlots = 16384;  // LDS counters: min(16, slots / k) priva
 constexpr int kFinThreads = 1024;  // hub_bucket_merge workgroup
 constexpr int kFinCap = 16384;     // LDS hash slots of hub_bucket_merge (128 KB)
 constexpr int kFinLds = 8192;      // buckets up to this many entries merge in LDS
-constexpr int kMaxBuckets = 2048;  // buckets per hub: min(4 x chunks, kMaxBuckets)
+constexpr int kMaxBuckets = 2048;  // buckets per hub: min(kBucketsPerChunk x chunks, kMaxBuckets)
 constexpr int kBucketsPerChunk = 1;  // one bucket per chunk: fewer, larger merge tasks measured faster than 4
 constexpr int kSmallThreads = 512;   // small bucket merge: <= kSmallMax entries in 32 KB of LDS
 constexpr int kSmallCap = 4096;
 constexpr int kSmallMax = 2048;
-constexpr int kHubDenseThreads = 1024;
 constexpr int kBigDegree = 64;    // moved nodes above this degree mark neighbours per workgroup
 
 struct Cand {
@@ -208,9 +208,24 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
         V[i] = 0;
     }
     __syncthreads();
-    for (int e = beg + lane; e < beg + deg; e += 64) {
-        const int lab = a.label[a.adjncy[e]];
-        const int w = a.adjwgt[e];
+    // All of a lane's (<= 8) edges are loaded first so that their label gathers are in flight
+    // together; the LDS insertions follow.
+    constexpr int EPL = kWaveCap / 2 / 64;  // edges per lane at the maximum degree (512)
+    int labs[EPL], ws[EPL];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+        const int e = beg + lane + 64 * j;
+        labs[j] = e < beg + deg ? a.adjncy[e] : -1;
+        ws[j] = e < beg + deg ? a.adjwgt[e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < EPL; ++j)
+        if (labs[j] >= 0) labs[j] = a.label[labs[j]];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+        const int lab = labs[j];
+        if (lab < 0) continue;
+        const int w = ws[j];
         if (DENSE) {
             atomicAdd(&V[lab], w);
             continue;
@@ -274,29 +289,68 @@ __device__ inline void block_reduce(int& conn_own, Cand& c) {
 }
 
 // ---------------------------------------------------------------- hubs, refinement: dense LDS
-// One workgroup per hub. Each wave adds into its own copy of the per-block counters (the k blocks
-// are few, so a single shared array would serialise on LDS atomics), then the copies are summed.
-__global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
-    LpArgs a, const int* __restrict__ list, int count, int num_labels) {
+// Per-wave copies of the per-block counters (the k blocks are few, so a single shared array
+// would serialise on LDS atomics), summed before the decision.
+// Chunked variant for refinement hubs of more than kHubChunk edges: one workgroup per chunk sums
+// its edges into per-wave LDS counters and adds the non-zero block sums to the hub's global
+// counters (dense_off[hub] + b); hub_dense_decide then decides. Single-chunk hubs decide here.
+__global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
+    LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt) {
     __shared__ int cnt[kHubDenseSlots];
-    const int gi = blockIdx.x;
-    if (gi >= count) return;  // uniform per block
+    __shared__ int hub;
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+        }
+        hub = lo;
+    }
+    __syncthreads();
+    const int gi = hub;
     const int v = list[gi];
     if (a.active[v] == 0) return;  // uniform per block
-    const int own = a.label[v];
+    const int nchunks = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int c_idx = chunk - chunk_prefix[gi];
     const int beg = a.xadj[v];
-    const int deg = a.xadj[v + 1] - beg;
-    const int copies = min(kHubDenseThreads / 64, kHubDenseSlots / num_labels);
+    const int end = a.xadj[v + 1];
+    const int e0 = beg + c_idx * kHubChunk;
+    const int e1 = min(end, e0 + kHubChunk);
+    const int copies = min(kHubThreads / 64, kHubDenseSlots / num_labels);
     int* mine = cnt + ((threadIdx.x >> 6) % copies) * num_labels;
-    for (int i = threadIdx.x; i < copies * num_labels; i += kHubDenseThreads) cnt[i] = 0;
+    for (int i = threadIdx.x; i < copies * num_labels; i += kHubThreads) cnt[i] = 0;
     __syncthreads();
-    for (int e = beg + threadIdx.x; e < beg + deg; e += kHubDenseThreads)
-        atomicAdd(&mine[a.label[a.adjncy[e]]], a.adjwgt[e]);
+    constexpr int EPT = kHubChunk / kHubThreads;
+    int labs[EPT], ws[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const int e = e0 + threadIdx.x + kHubThreads * j;
+        labs[j] = e < e1 ? a.adjncy[e] : -1;
+        ws[j] = e < e1 ? a.adjwgt[e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+        if (labs[j] >= 0) labs[j] = a.label[labs[j]];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+        if (labs[j] >= 0) atomicAdd(&mine[labs[j]], ws[j]);
     __syncthreads();
+    if (nchunks > 1) {
+        int* out = dcnt + dense_off[gi];
+        for (int b = threadIdx.x; b < num_labels; b += kHubThreads) {
+            int conn = 0;
+            for (int j = 0; j < copies; ++j) conn += cnt[j * num_labels + b];
+            if (conn) atomicAdd(&out[b], conn);
+        }
+        return;
+    }
+    const int own = a.label[v];
     const int nw = a.vwgt[v];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
+    for (int b = threadIdx.x; b < num_labels; b += kHubThreads) {
         int conn = 0;
         for (int j = 0; j < copies; ++j) conn += cnt[j * num_labels + b];
         if (conn == 0) continue;
@@ -309,10 +363,44 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense(
         const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
         if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
     }
-    block_reduce<kHubDenseThreads>(conn_own, c);
+    block_reduce<kHubThreads>(conn_own, c);
     if (threadIdx.x == 0) {
         decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
-        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(end - beg));
+    }
+}
+
+// One workgroup per multi-chunk refinement hub: decides from the summed counters and clears them.
+__global__ __launch_bounds__(256) void hub_dense_decide(
+    LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
+    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt) {
+    const int mi = blockIdx.x;
+    if (mi >= n_multi) return;
+    const int gi = multi[mi];
+    const int v = list[gi];
+    if (a.active[v] == 0) return;  // uniform per block
+    int* cnt = dcnt + dense_off[gi];
+    const int own = a.label[v];
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (int b = threadIdx.x; b < num_labels; b += 256) {
+        const int conn = cnt[b];
+        if (conn == 0) continue;
+        cnt[b] = 0;
+        if (b == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[b];
+        if ((long long)sz + nw >= (long long)bound_of(a, b)) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
+        if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
+    }
+    block_reduce<256>(conn_own, c);
+    if (threadIdx.x == 0) {
+        decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
+        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
     }
 }
 
@@ -352,9 +440,23 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
         V[i] = 0;
     }
     __syncthreads();
-    for (int e = e0 + threadIdx.x; e < e1; e += kHubThreads) {
-        const int lab = a.label[a.adjncy[e]];
-        const int w = a.adjwgt[e];
+    // load every edge of the thread first (independent label gathers in flight), then insert
+    constexpr int EPT = kHubChunk / kHubThreads;
+    int labs[EPT], ws[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const int e = e0 + threadIdx.x + kHubThreads * j;
+        labs[j] = e < e1 ? a.adjncy[e] : -1;
+        ws[j] = e < e1 ? a.adjwgt[e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+        if (labs[j] >= 0) labs[j] = a.label[labs[j]];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const int lab = labs[j];
+        if (lab < 0) continue;
+        const int w = ws[j];
         uint32_t h = mix32((uint32_t)lab) & (uint32_t)(kHubCap - 1);
         while (true) {
             const int old = atomicCAS(&K[h], -1, lab);
@@ -916,7 +1018,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     }
     hub_total_ = 0;
     for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = 0;
-    if (hubs == 0 || dense_hub_) {
+    if (hubs == 0) {
         KH_CHECK(hipGetLastError());
         return;
     }
@@ -931,6 +1033,46 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     std::vector<int> deg(hubs);
     KH_CHECK(hipMemcpyAsync(deg.data(), hub_deg_.p, sizeof(int) * hubs, hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
+    if (dense_hub_) {
+        // chunked dense refinement: chunk prefix per sub-round and global counters for every
+        // hub of more than one chunk
+        std::vector<int> prefix, multi, doff(hubs, -1);
+        long long dtotal = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            chunk_base_[sr] = (int)prefix.size();
+            multi_base_[sr] = (int)multi.size();
+            int c = 0;
+            for (int i = 0; i < hub_n_[sr]; ++i) {
+                const int nch = (deg[hub_base_[sr] + i] + kHubChunk - 1) / kHubChunk;
+                prefix.push_back(c);
+                c += nch;
+                if (nch > 1) {
+                    doff[hub_base_[sr] + i] = (int)dtotal;
+                    dtotal += num_labels;
+                    multi.push_back(i);
+                }
+            }
+            prefix.push_back(c);
+            hub_chunks_[sr] = c;
+            hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
+        }
+        KH_REQUIRE(dtotal < (1LL << 31), "too many refinement hub counters");
+        chunk_prefix_.reserve((long long)prefix.size());
+        multi_.reserve(std::max<long long>((long long)multi.size(), 1));
+        dense_off_.reserve(hubs);
+        dense_cnt_.reserve(std::max<long long>(dtotal, 1));
+        fill_async(dense_cnt_.p, std::max<long long>(dtotal, 1), 0, s);
+        KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
+                                hipMemcpyHostToDevice, s));
+        if (!multi.empty())
+            KH_CHECK(hipMemcpyAsync(multi_.p, multi.data(), sizeof(int) * multi.size(),
+                                    hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(dense_off_.p, doff.data(), sizeof(int) * hubs,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipStreamSynchronize(s));  // the host vectors must outlive the copies
+        KH_CHECK(hipGetLastError());
+        return;
+    }
     std::vector<int> prefix, multi, tasks;
     std::vector<long long> toff(hubs, 0), tcap(hubs, 0), obase(hubs, 0);
     std::vector<int> ctb(hubs, 0);
@@ -1113,7 +1255,14 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         break;
                     default: {
                         if (!cl && dense_hub_) {
-                            hipLaunchKernelGGL(propose_hub_dense, dim3(cnt), dim3(kHubDenseThreads), 0, s, a, lst, cnt, nl);
+                            const int* cp = chunk_prefix_.p + chunk_base_[sr];
+                            const int* dof = dense_off_.p + hub_base_[sr];
+                            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p);
+                            ps.close();
+                            if (hub_multi_n_[sr] > 0) {
+                                ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_multi_n_[sr], s);
+                                hipLaunchKernelGGL(hub_dense_decide, dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, multi_.p + multi_base_[sr], hub_multi_n_[sr], nl, dof, dense_cnt_.p);
+                            }
                             break;
                         }
                         const int* cp = chunk_prefix_.p + chunk_base_[sr];

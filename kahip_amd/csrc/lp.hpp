@@ -97,6 +97,7 @@ private:
     DevBuf<unsigned long long> top_, dtop_;
     DevBuf<uint8_t> active_, next_active_;
     DevBuf<int> hub_key_, hub_val_;
+    DevBuf<int> dense_off_, dense_cnt_;  // chunked dense refinement hubs: per-hub counters
     long long hub_total_ = 0;
 };
 

@@ -24,9 +24,56 @@ def ip_seed(seed):
     return ((seed & 0xFFFFFFFF) * 0x9E3779B97F4A7C15 + 0x1234567) & 0xFFFFFFFFFFFFFFFF
 
 
+def reorder_wanted(xadj, mode=-1):
+    n = len(xadj) - 1
+    if mode == 0 or n == 0:
+        return False
+    if mode > 0:
+        return True
+    deg = np.diff(np.asarray(xadj, np.int64))
+    return int(deg.max()) * n > 32 * int(xadj[-1])
+
+
+def reorder_by_degree(xadj, adjncy, vwgt, adjwgt):
+    """Node i of the result = the i-th node by (degree desc, id asc); rows keep their edge order
+    with mapped ids. Returns (graph, inv) with inv[old] = new."""
+    x = np.asarray(xadj, np.int64)
+    deg = np.diff(x)
+    order = np.argsort(-deg, kind="stable")
+    inv = np.empty(len(deg), np.int64)
+    inv[order] = np.arange(len(deg))
+    nx = np.concatenate([[0], np.cumsum(deg[order])])
+    # old edge index of every new entry: rows of order[] concatenated
+    starts = np.repeat(x[order] - nx[:-1], deg[order])
+    old_e = np.arange(int(nx[-1])) + starts
+    na = inv[np.asarray(adjncy, np.int64)[old_e]].astype(np.int32)
+    nw = None if adjwgt is None else np.asarray(adjwgt, np.int32)[old_e]
+    nv = None if vwgt is None else np.asarray(vwgt, np.int32)[order]
+    return (nx.astype(np.int32), na, nv, nw), inv
+
+
 def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, cluster_rounds=10,
                      refine_rounds=25, subrounds=4, cluster_factor=18, stop_factor=5000,
-                     ip=None, scores=None, rating_levels=64):
+                     ip=None, scores=None, rating_levels=64, reorder=-1):
+    """As mirror_partition_plain, after the library's degree renumbering of skewed inputs
+    (reorder: -1 automatic, 0 off, 1 on); the partition is returned in the input numbering."""
+    if reorder_wanted(xadj, reorder):
+        g, inv = reorder_by_degree(xadj, adjncy, vwgt, adjwgt)
+        part, info = mirror_partition_plain(*g, k, imbalance, seed, cluster_rounds,
+                                            refine_rounds, subrounds, cluster_factor,
+                                            stop_factor, ip, scores, rating_levels)
+        info["reordered"] = True
+        return part[inv], info
+    part, info = mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance, seed,
+                                        cluster_rounds, refine_rounds, subrounds, cluster_factor,
+                                        stop_factor, ip, scores, rating_levels)
+    info["reordered"] = False
+    return part, info
+
+
+def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0,
+                           cluster_rounds=10, refine_rounds=25, subrounds=4, cluster_factor=18,
+                           stop_factor=5000, ip=None, scores=None, rating_levels=64):
     """Returns (part, info). `ip(xadj, adjncy, vwgt, adjwgt, k, U, reps, seed)` partitions the
     coarsest graph (the library's kahip_amd.initial_partition by default). `scores(xadj, adjncy,
     vwgt, adjwgt, B, rating_seed)` (optional) gives the int32 edge scores that replace the edge

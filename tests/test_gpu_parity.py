@@ -173,6 +173,18 @@ def test_pipeline_matches_mirror(G, name, k, seed):
     assert st["upper_bound"] == info["U"]
 
 
+@pytest.mark.parametrize("name,k,seed,reorder", [("rmat12", 8, 4, 0), ("rgg15", 2, 0, 1),
+                                                  ("star1500", 4, 1, -1), ("er200w", 2, 3, 1)])
+def test_pipeline_reorder_modes_match_mirror(G, name, k, seed, reorder):
+    x, a, vw, ew = G[name]
+    part, st = K.partition(x, a, vw, ew, k=k, imbalance=0.03, seed=seed, reorder=reorder)
+    want, info = mirror_partition(x, a, vw, ew, k, 0.03, seed, reorder=reorder)
+    assert st["reordered"] == info["reordered"]
+    assert st["level_n"] == info["level_n"]
+    assert np.array_equal(part, want)
+    assert st["cut"] == O.edge_cut(x, a, ew, want)
+
+
 def test_pipeline_deterministic(G):
     x, a, vw, ew = G["rmat12"]
     p1, s1 = K.partition(x, a, vw, ew, k=16, seed=5)
