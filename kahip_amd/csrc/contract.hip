@@ -35,6 +35,7 @@ namespace {
 constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
 constexpr int kRowMaxDistinct = 512;
 constexpr int kRowSmall = 128;    // rows with summed member degree <= this use 256 slots
+constexpr int kDirectGlobal = 4096;  // rows with more member edges go to the chunked tier
 constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
 constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
 constexpr int kBigMaxDistinct = 4096;
@@ -86,24 +87,29 @@ __global__ void scatter_members(int n, const int* __restrict__ map, const int* _
 // (it cannot overflow), the rest to the kRowCap variant. Workgroup-aggregated list appends.
 __global__ __launch_bounds__(256) void row_bins(int nc, const int* __restrict__ fdeg,
                                                 int* __restrict__ small, int* __restrict__ mid,
-                                                int* __restrict__ counts) {
-    __shared__ int ls, lm, bs, bm;
-    if (threadIdx.x == 0) ls = lm = 0;
+                                                int* __restrict__ counts,
+                                                int* __restrict__ global_list,
+                                                int* __restrict__ global_count) {
+    __shared__ int ls, lm, lg, bs, bm, bg;
+    if (threadIdx.x == 0) ls = lm = lg = 0;
     __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1, rank = 0;
     if (p < nc) {
-        cls = fdeg[p] <= kRowSmall ? 0 : 1;
-        rank = atomicAdd(cls == 0 ? &ls : &lm, 1);
+        const int fd = fdeg[p];
+        cls = fd <= kRowSmall ? 0 : (fd <= kDirectGlobal ? 1 : 2);
+        rank = atomicAdd(cls == 0 ? &ls : (cls == 1 ? &lm : &lg), 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         bs = ls ? atomicAdd(&counts[0], ls) : 0;
         bm = lm ? atomicAdd(&counts[1], lm) : 0;
+        bg = lg ? atomicAdd(global_count, lg) : 0;
     }
     __syncthreads();
     if (cls == 0) small[bs + rank] = p;
     else if (cls == 1) mid[bm + rank] = p;
+    else if (cls == 2) global_list[bg + rank] = p;
 }
 
 // One wave per coarse row. The members' edges are expanded with a load-balanced scan (64
@@ -385,29 +391,46 @@ __global__ void member_degrees(int n, const int* __restrict__ members,
 }
 
 // Rows beyond the LDS tiers, split into chunks of kGChunk member edges (the row's member edges
-// flattened through the member-degree prefix mpre, which equals tpos at the row's first member).
-// Each chunk aggregates (q -> sum w, min e) in LDS and adds its entries into the row's global
-// table, one atomic triple per distinct neighbour of the chunk.
-__global__ __launch_bounds__(kGThreads) void rows_global_chunks(
+// flattened through the member-degree prefix mpre). Row i has P_i = min(chunks, kGMaxBuckets)
+// buckets of neighbour clusters q (bucket = gbucket(q, P_i)).
+//   gchunk   each chunk aggregates (q -> sum w, min e) in LDS and writes its entries grouped by
+//            bucket (per-chunk bucket offsets offm, bucket-major counts cntT);
+//   gscatter after a scan of cntT, moves them so that every (row, bucket) is one range;
+//   gmerge   one workgroup per (row, bucket) sums the range in LDS and appends the merged
+//            entries (key min e, value (q, w)) to the row's output run; ranges of more than
+//            kGMergeMax entries are queued for gmerge_global (a private global table per task).
+// No global atomics on shared tables: every merge is local to one workgroup.
+constexpr int kGMaxBuckets = 1024;
+constexpr int kGMergeCap = 8192;  // LDS slots of gmerge
+constexpr int kGMergeMax = kGMergeCap / 2;
+
+__device__ inline int gbucket(int q, int P) {
+    return (int)(mix32((uint32_t)q ^ 0x27d4eb2fu) % (uint32_t)P);
+}
+
+__device__ inline int row_of_chunk(const int* __restrict__ chunk_prefix, int count, int chunk) {
+    int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kGThreads) void gchunk(
     const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
     const int* __restrict__ xadj, const int* __restrict__ adjncy, const int* __restrict__ adjwgt,
     const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
-    const int* __restrict__ mpre, const long long* __restrict__ hoff,
-    const long long* __restrict__ hcap, int* __restrict__ HK, int* __restrict__ HV,
-    int* __restrict__ HM) {
+    const int* __restrict__ mpre, const long long* __restrict__ offm_base, int* __restrict__ offm,
+    const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials) {
     __shared__ int K[kGCap];
     __shared__ int V[kGCap];
     __shared__ int M[kGCap];
+    __shared__ int bcur[kGMaxBuckets + 1];
+    __shared__ int wsum[kGThreads / 64];
     __shared__ int row;
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = count;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
-        }
-        row = lo;
-    }
+    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
     for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
         K[i] = -1;
         V[i] = 0;
@@ -416,16 +439,20 @@ __global__ __launch_bounds__(kGThreads) void rows_global_chunks(
     __syncthreads();
     const int ri = row;
     const int p = list[ri];
+    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    const int c_idx = chunk - chunk_prefix[ri];
+    const int P = min(nch, kGMaxBuckets);
     const int mb = moff[p], me = moff[p + 1];
     const long long f_row = mpre[mb];
     const long long f_end = mpre[me];
-    const long long f0 = f_row + (long long)(chunk - chunk_prefix[ri]) * kGChunk;
+    const long long f0 = f_row + (long long)c_idx * kGChunk;
     const long long f1 = min(f_end, f0 + kGChunk);
-    // each thread walks a contiguous run of the chunk
-    const long long per = (kGChunk + kGThreads - 1) / kGThreads;
-    long long a0 = f0 + threadIdx.x * per;
-    const long long a1 = min(f1, a0 + per);
-    if (a0 < a1) {
+    constexpr int PER = kGChunk / kGThreads;  // a contiguous run of PER member edges per thread
+    const long long a0 = f0 + (long long)threadIdx.x * PER;
+    int es[PER], qs[PER], ws[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) es[j] = -1;
+    if (a0 < f1) {
         int lo = mb, hi = me;  // largest member index with mpre <= a0
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -435,79 +462,250 @@ __global__ __launch_bounds__(kGThreads) void rows_global_chunks(
         int u = members[mi];
         long long mstart = mpre[mi];
         long long mend = mstart + (xadj[u + 1] - xadj[u]);
-        for (long long f = a0; f < a1; ++f) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const long long f = a0 + j;
+            if (f >= f1) break;
             while (f >= mend) {
                 ++mi;
                 u = members[mi];
                 mstart = mpre[mi];
                 mend = mstart + (xadj[u + 1] - xadj[u]);
             }
-            const int e = xadj[u] + (int)(f - mstart);
-            const int q = map[adjncy[e]];
-            if (q == p) continue;
-            uint32_t h = mix32((uint32_t)q) & (uint32_t)(kGCap - 1);
-            while (true) {
-                const int old = atomicCAS(&K[h], -1, q);
-                if (old == -1 || old == q) {
-                    atomicAdd(&V[h], adjwgt[e]);
-                    atomicMin(&M[h], e);
-                    break;
-                }
-                h = (h + 1) & (uint32_t)(kGCap - 1);
-            }
+            es[j] = xadj[u] + (int)(f - mstart);
         }
     }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
+        ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+        if (es[j] >= 0) qs[j] = map[qs[j]];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int q = qs[j];
+        if (es[j] < 0 || q == p) continue;
+        uint32_t h = mix32((uint32_t)q) & (uint32_t)(kGCap - 1);
+        while (true) {
+            const int old = atomicCAS(&K[h], -1, q);
+            if (old == -1 || old == q) {
+                atomicAdd(&V[h], ws[j]);
+                atomicMin(&M[h], es[j]);
+                break;
+            }
+            h = (h + 1) & (uint32_t)(kGCap - 1);
+        }
+    }
+    for (int i = threadIdx.x; i <= P; i += kGThreads) bcur[i] = 0;
     __syncthreads();
-    const long long cap = hcap[ri];
-    int* GK = HK + hoff[ri];
-    int* GV = HV + hoff[ri];
-    int* GM = HM + hoff[ri];
+    for (int i = threadIdx.x; i < kGCap; i += kGThreads)
+        if (K[i] >= 0) atomicAdd(&bcur[gbucket(K[i], P)], 1);
+    __syncthreads();
+    // exclusive scan of bcur[0..P): each thread owns a run of buckets
+    const int per = (P + kGThreads - 1) / kGThreads;
+    const int b0 = threadIdx.x * per, b1 = min(P, b0 + per);
+    int run = 0;
+    for (int b = b0; b < b1; ++b) run += bcur[b];
+    int incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if ((threadIdx.x & 63) >= o) incl += y;
+    }
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < (threadIdx.x >> 6); ++w) wbase += wsum[w];
+    int acc = wbase + incl - run;
+    for (int b = b0; b < b1; ++b) {
+        const int cnt = bcur[b];
+        bcur[b] = acc;
+        acc += cnt;
+    }
+    __syncthreads();
+    int* offs = offm + offm_base[ri] + (long long)c_idx * (P + 1);
+    int* cnt_t = cntT + ct_base[ri] + c_idx;  // bucket-major: [b * nch + c_idx]
+    int tot = 0;
+    for (int w = 0; w < kGThreads / 64; ++w) tot += wsum[w];
+    for (int b = threadIdx.x; b < P; b += kGThreads) {
+        offs[b] = bcur[b];
+        cnt_t[(long long)b * nch] = (b + 1 < P ? bcur[b + 1] : tot) - bcur[b];
+    }
+    if (threadIdx.x == 0) offs[P] = tot;
+    __syncthreads();
+    int4* out = partials + (long long)chunk * kGChunk;
     for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
         const int q = K[i];
         if (q < 0) continue;
-        long long h = (long long)(mix32((uint32_t)q) & (uint32_t)(cap - 1));
+        out[atomicAdd(&bcur[gbucket(q, P)], 1)] = make_int4(q, V[i], M[i], 0);
+    }
+}
+
+__global__ __launch_bounds__(256) void gscatter(
+    const int* __restrict__ chunk_prefix, int count, const long long* __restrict__ offm_base,
+    const int* __restrict__ offm, const long long* __restrict__ ct_base,
+    const int* __restrict__ posT, const int4* __restrict__ partials,
+    int4* __restrict__ bucketed) {
+    __shared__ int row;
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
+    __syncthreads();
+    const int ri = row;
+    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    const int c_idx = chunk - chunk_prefix[ri];
+    const int P = min(nch, kGMaxBuckets);
+    const int* offs = offm + offm_base[ri] + (long long)c_idx * (P + 1);
+    const int* pos = posT + ct_base[ri] + c_idx;
+    const int4* src = partials + (long long)chunk * kGChunk;
+    const int total = offs[P];
+    for (int i = threadIdx.x; i < total; i += 256) {
+        const int4 e = src[i];
+        const int b = gbucket(e.x, P);
+        bucketed[pos[(long long)b * nch] + (i - offs[b])] = e;
+    }
+}
+
+__global__ __launch_bounds__(512) void gmerge(
+    const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix, int count,
+    int n_tasks, const long long* __restrict__ ct_base, const int* __restrict__ posT,
+    const int4* __restrict__ bucketed, const int* __restrict__ obase, int* __restrict__ rowcnt,
+    unsigned* __restrict__ okey, unsigned long long* __restrict__ oval,
+    int* __restrict__ big_tasks, int* __restrict__ n_big) {
+    __shared__ int K[kGMergeCap];
+    __shared__ int V[kGMergeCap];
+    __shared__ int M[kGMergeCap];
+    __shared__ int row, nout, base;
+    const int task = blockIdx.x;
+    if (task >= n_tasks) return;  // uniform
+    if (threadIdx.x == 0) {
+        row = row_of_chunk(task_prefix, count, task);
+        nout = 0;
+    }
+    __syncthreads();
+    const int ri = row;
+    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    const int b = task - task_prefix[ri];
+    const int r0 = posT[ct_base[ri] + (long long)b * nch];
+    const int r1 = posT[ct_base[ri] + (long long)(b + 1) * nch];
+    const int total = r1 - r0;
+    if (total > kGMergeMax) {
+        if (threadIdx.x == 0) big_tasks[atomicAdd(n_big, 1)] = task;
+        return;  // uniform
+    }
+    int cap = 64;
+    while (cap < 2 * total) cap <<= 1;
+    for (int i = threadIdx.x; i < cap; i += 512) {
+        K[i] = -1;
+        V[i] = 0;
+        M[i] = 0x7fffffff;
+    }
+    __syncthreads();
+    for (int i = r0 + threadIdx.x; i < r1; i += 512) {
+        const int4 e = bucketed[i];
+        uint32_t h = mix32((uint32_t)e.x) & (uint32_t)(cap - 1);
         while (true) {
-            const int old = atomicCAS(&GK[h], -1, q);
-            if (old == -1 || old == q) {
-                atomicAdd(&GV[h], V[i]);
-                atomicMin(&GM[h], M[i]);
+            const int old = atomicCAS(&K[h], -1, e.x);
+            if (old == -1 || old == e.x) {
+                atomicAdd(&V[h], e.y);
+                atomicMin(&M[h], e.z);
+                break;
+            }
+            h = (h + 1) & (uint32_t)(cap - 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cap; i += 512)
+        if (K[i] >= 0) atomicAdd(&nout, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) base = obase[ri] + atomicAdd(&rowcnt[ri], nout);
+    if (threadIdx.x == 0) nout = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cap; i += 512) {
+        const int q = K[i];
+        if (q < 0) continue;
+        const int at = base + atomicAdd(&nout, 1);
+        okey[at] = (unsigned)M[i];
+        oval[at] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[i];
+    }
+}
+
+// Oversized (row, bucket) ranges: one workgroup each with a private global hash table of
+// tcap[t] slots at toff[t] (cleared beforehand), then the same append as gmerge.
+__global__ __launch_bounds__(1024) void gmerge_global(
+    const int* __restrict__ big_tasks, int n_big, const int* __restrict__ chunk_prefix,
+    const int* __restrict__ task_prefix, int count, const long long* __restrict__ ct_base,
+    const int* __restrict__ posT, const int4* __restrict__ bucketed,
+    const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
+    int* __restrict__ TV, int* __restrict__ TM, const int* __restrict__ obase,
+    int* __restrict__ rowcnt, unsigned* __restrict__ okey, unsigned long long* __restrict__ oval) {
+    __shared__ int row, nout, base;
+    const int t = blockIdx.x;
+    if (t >= n_big) return;
+    const int task = big_tasks[t];
+    if (threadIdx.x == 0) {
+        row = row_of_chunk(task_prefix, count, task);
+        nout = 0;
+    }
+    __syncthreads();
+    const int ri = row;
+    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    const int b = task - task_prefix[ri];
+    const int r0 = posT[ct_base[ri] + (long long)b * nch];
+    const int r1 = posT[ct_base[ri] + (long long)(b + 1) * nch];
+    const long long cap = tcap[t];
+    int* K = TK + toff[t];
+    int* V = TV + toff[t];
+    int* Mn = TM + toff[t];
+    for (int i = r0 + threadIdx.x; i < r1; i += 1024) {
+        const int4 e = bucketed[i];
+        long long h = (long long)(mix32((uint32_t)e.x) & (uint32_t)(cap - 1));
+        while (true) {
+            const int old = atomicCAS(&K[h], -1, e.x);
+            if (old == -1 || old == e.x) {
+                atomicAdd(&V[h], e.y);
+                atomicMin(&Mn[h], e.z);
                 break;
             }
             h = (h + 1) & (cap - 1);
         }
     }
+    __threadfence();  // the table's atomics complete before it is read back (rare path)
+    __syncthreads();
+    for (long long i = threadIdx.x; i < cap; i += 1024)
+        if (K[i] >= 0) atomicAdd(&nout, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) base = obase[ri] + atomicAdd(&rowcnt[ri], nout);
+    if (threadIdx.x == 0) nout = 0;
+    __syncthreads();
+    for (long long i = threadIdx.x; i < cap; i += 1024) {
+        const int q = K[i];
+        if (q < 0) continue;
+        const int at = base + atomicAdd(&nout, 1);
+        okey[at] = (unsigned)Mn[i];
+        oval[at] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[i];
+    }
 }
 
-// One workgroup per row beyond the LDS tiers: moves the row's table entries to the sort buffers
-// (key = min fine edge, value = (q, w)) and clears the table.
-__global__ __launch_bounds__(256) void rows_global_extract(
-    const int* __restrict__ list, int count, const long long* __restrict__ hoff,
-    const long long* __restrict__ hcap, const int* __restrict__ obase, int* __restrict__ HK,
-    int* __restrict__ HV, int* __restrict__ HM, int* __restrict__ cdeg,
-    unsigned* __restrict__ okey, unsigned long long* __restrict__ oval) {
-    __shared__ int n_out;
-    const int i = blockIdx.x;
-    if (i >= count) return;
-    const int p = list[i];
-    const long long cap = hcap[i];
-    int* K = HK + hoff[i];
-    int* V = HV + hoff[i];
-    int* Mn = HM + hoff[i];
-    if (threadIdx.x == 0) n_out = 0;
-    __syncthreads();
-    const int base = obase[i];
-    for (long long h = threadIdx.x; h < cap; h += 256) {
-        const int q = K[h];
-        if (q < 0) continue;
-        const int idx = atomicAdd(&n_out, 1);
-        okey[base + idx] = (unsigned)Mn[h];
-        oval[base + idx] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[h];
-        K[h] = -1;
-        V[h] = 0;
-        Mn[h] = 0x7fffffff;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) cdeg[p] = n_out;
+__global__ void row_counts_to_cdeg(const int* __restrict__ list, int count,
+                                   const int* __restrict__ rowcnt, int* __restrict__ cdeg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) cdeg[list[i]] = rowcnt[i];
+}
+
+__global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
+                                const int* __restrict__ chunk_prefix,
+                                const int* __restrict__ task_prefix, int count,
+                                const long long* __restrict__ ct_base,
+                                const int* __restrict__ posT, int* __restrict__ totals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_big) return;
+    const int task = big_tasks[t];
+    const int ri = row_of_chunk(task_prefix, count, task);
+    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    const int b = task - task_prefix[ri];
+    totals[t] = posT[ct_base[ri] + (long long)(b + 1) * nch] - posT[ct_base[ri] + (long long)b * nch];
 }
 
 __global__ void overflow_ends(const int* __restrict__ list, int count, const int* __restrict__ obase,
@@ -643,7 +841,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         rcount_.reserve(2);
         fill_async(rcount_.p, 2, 0, s);
         hipLaunchKernelGGL(row_bins, dim3(grid_for(nc, 256)), dim3(256), 0, s, nc, fdeg_.p,
-                           rsmall_.p, rmid_.p, rcount_.p);
+                           rsmall_.p, rmid_.p, rcount_.p, ovl2_list_.p, ovl_count_.p + 1);
         int rc[2] = {0, 0};
         KH_CHECK(hipMemcpyAsync(rc, rcount_.p, sizeof(rc), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -678,62 +876,121 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         ofd_.reserve(novl);
         hipLaunchKernelGGL(hub_cap_kernel, dim3(grid_for(novl, 256)), dim3(256), 0, s,
                            ovl2_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p, ofd_.p);
-        std::vector<long long> caps(novl), offs(novl);
-        std::vector<int> bounds(novl), bases(novl);
-        KH_CHECK(hipMemcpyAsync(caps.data(), hcap_.p, sizeof(long long) * novl,
-                                hipMemcpyDeviceToHost, s));
+        std::vector<int> bounds(novl), bases(novl), fd(novl);
         KH_CHECK(hipMemcpyAsync(bounds.data(), obound_.p, sizeof(int) * novl,
                                 hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipMemcpyAsync(fd.data(), ofd_.p, sizeof(int) * novl, hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
-        long long htotal = 0, ototal = 0;
+        // per row: chunks of its member edges, buckets, count / offset table bases, tasks
+        std::vector<int> cpre(novl + 1), tpre(novl + 1);
+        std::vector<long long> ofb(novl), ctb(novl);
+        long long ototal = 0, offm_total = 0, ct_total = 0;
+        int nchunks = 0, ntasks = 0;
         for (int i = 0; i < novl; ++i) {
-            offs[i] = htotal;
-            htotal += caps[i];
             bases[i] = (int)ototal;
             ototal += bounds[i];
+            const int nch = std::max(1, (fd[i] + kGChunk - 1) / kGChunk);
+            const int P = std::min(nch, kGMaxBuckets);
+            cpre[i] = nchunks;
+            tpre[i] = ntasks;
+            ofb[i] = offm_total;
+            ctb[i] = ct_total;
+            nchunks += nch;
+            ntasks += P;
+            offm_total += (long long)nch * (P + 1);
+            ct_total += (long long)nch * P;
         }
-        KH_REQUIRE(ototal < (1LL << 31), "contraction scratch exceeds 2^31 entries");
-        KH_CHECK(hipMemcpyAsync(hoff_.p, offs.data(), sizeof(long long) * novl,
+        cpre[novl] = nchunks;
+        tpre[novl] = ntasks;
+        KH_REQUIRE(ototal < (1LL << 31) && ct_total < (1LL << 31),
+                   "contraction scratch exceeds 2^31 entries");
+        obase_.reserve(novl + 1);
+        gchunk_prefix_.reserve(novl + 1);
+        gtask_prefix_.reserve(novl + 1);
+        goffm_base_.reserve(novl);
+        gct_base_.reserve(novl);
+        KH_CHECK(hipMemcpyAsync(obase_.p, bases.data(), sizeof(int) * novl, hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(gchunk_prefix_.p, cpre.data(), sizeof(int) * (novl + 1),
                                 hipMemcpyHostToDevice, s));
-        KH_CHECK(hipMemcpyAsync(obase_.p, bases.data(), sizeof(int) * novl,
+        KH_CHECK(hipMemcpyAsync(gtask_prefix_.p, tpre.data(), sizeof(int) * (novl + 1),
                                 hipMemcpyHostToDevice, s));
-        hk_.reserve(htotal);
-        hv_.reserve(htotal);
-        hm_.reserve(htotal);
-        fill_async(hk_.p, htotal, -1, s);
-        fill_async(hv_.p, htotal, 0, s);
-        fill_async(hm_.p, htotal, 0x7fffffff, s);
+        KH_CHECK(hipMemcpyAsync(goffm_base_.p, ofb.data(), sizeof(long long) * novl,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(gct_base_.p, ctb.data(), sizeof(long long) * novl,
+                                hipMemcpyHostToDevice, s));
         okey_.reserve(ototal);
         oval_.reserve(ototal);
         okey2_.reserve(ototal);
         oval2_.reserve(ototal);
-        // chunks of the member-edge range of each row (row capacities fdeg are on the host now)
-        std::vector<int> fd(novl), cpre(novl + 1);
-        KH_CHECK(hipMemcpyAsync(fd.data(), ofd_.p, sizeof(int) * novl, hipMemcpyDeviceToHost, s));
-        KH_CHECK(hipStreamSynchronize(s));
-        int nchunks = 0;
-        for (int i = 0; i < novl; ++i) {
-            cpre[i] = nchunks;
-            nchunks += (fd[i] + kGChunk - 1) / kGChunk;
-        }
-        cpre[novl] = nchunks;
-        gchunk_prefix_.reserve(novl + 1);
-        KH_CHECK(hipMemcpyAsync(gchunk_prefix_.p, cpre.data(), sizeof(int) * (novl + 1),
-                                hipMemcpyHostToDevice, s));
+        goffm_.reserve(offm_total);
+        gcnt_.reserve(ct_total + 1);
+        gpos_.reserve(ct_total + 1);
+        gpart_.reserve((long long)nchunks * kGChunk);
+        gbuck_.reserve((long long)nchunks * kGChunk);
+        growcnt_.reserve(novl);
+        gbig_.reserve(std::max(ntasks, 1));
+        gnbig_.reserve(1);
+        fill_async(gcnt_.p + ct_total, 1, 0, s);  // sentinel
+        fill_async(growcnt_.p, novl, 0, s);
+        fill_async(gnbig_.p, 1, 0, s);
         mdeg_.reserve((long long)n + 1);
         mpre_.reserve((long long)n + 1);
         hipLaunchKernelGGL(member_degrees, dim3(grid_for(n, 256)), dim3(256), 0, s, n, members_.p,
                            g.xadj, mdeg_.p);
         fill_async(mdeg_.p + n, 1, 0, s);
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
-        if (nchunks > 0)
-            hipLaunchKernelGGL(rows_global_chunks, dim3(nchunks), dim3(kGThreads), 0, s,
-                               ovl2_list_.p, novl, gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt,
-                               map, moff_.p, members_.p, mpre_.p, hoff_.p, hcap_.p, hk_.p, hv_.p,
-                               hm_.p);
-        hipLaunchKernelGGL(rows_global_extract, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl,
-                           hoff_.p, hcap_.p, obase_.p, hk_.p, hv_.p, hm_.p, cdeg_.p, okey_.p,
-                           oval_.p);
+        hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
+                           gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
+                           mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p, gpart_.p);
+        scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
+        hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,
+                           goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
+        hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(512), 0, s, gchunk_prefix_.p,
+                           gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p, obase_.p,
+                           growcnt_.p, okey_.p, oval_.p, gbig_.p, gnbig_.p);
+        int nbig = 0;
+        KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        KH_CHECK(hipStreamSynchronize(s));
+        if (nbig > 0) {
+            // rare: a (row, bucket) range too large for LDS gets a private global table
+            DevBuf<int> totals;
+            totals.reserve(nbig);
+            hipLaunchKernelGGL(big_task_totals, dim3(grid_for(nbig, 256)), dim3(256), 0, s,
+                               gbig_.p, nbig, gchunk_prefix_.p, gtask_prefix_.p, novl,
+                               gct_base_.p, gpos_.p, totals.p);
+            std::vector<int> tot(nbig);
+            KH_CHECK(hipMemcpyAsync(tot.data(), totals.p, sizeof(int) * nbig,
+                                    hipMemcpyDeviceToHost, s));
+            KH_CHECK(hipStreamSynchronize(s));
+            std::vector<long long> toff(nbig), tcap(nbig);
+            long long ttotal = 0;
+            for (int t = 0; t < nbig; ++t) {
+                long long c = 64;
+                while (c < 2LL * tot[t]) c <<= 1;
+                toff[t] = ttotal;
+                tcap[t] = c;
+                ttotal += c;
+            }
+            hcap_.reserve(nbig);
+            hoff_.reserve(nbig);
+            KH_CHECK(hipMemcpyAsync(hoff_.p, toff.data(), sizeof(long long) * nbig,
+                                    hipMemcpyHostToDevice, s));
+            KH_CHECK(hipMemcpyAsync(hcap_.p, tcap.data(), sizeof(long long) * nbig,
+                                    hipMemcpyHostToDevice, s));
+            hk_.reserve(ttotal);
+            hv_.reserve(ttotal);
+            hm_.reserve(ttotal);
+            fill_async(hk_.p, ttotal, -1, s);
+            fill_async(hv_.p, ttotal, 0, s);
+            fill_async(hm_.p, ttotal, 0x7fffffff, s);
+            hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
+                               gchunk_prefix_.p, gtask_prefix_.p, novl, gct_base_.p, gpos_.p,
+                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, obase_.p,
+                               growcnt_.p, okey_.p, oval_.p);
+            KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
+        }
+        hipLaunchKernelGGL(row_counts_to_cdeg, dim3(grid_for(novl, 256)), dim3(256), 0, s,
+                           ovl2_list_.p, novl, growcnt_.p, cdeg_.p);
         hipLaunchKernelGGL(overflow_ends, dim3(grid_for(novl, 256)), dim3(256), 0, s,
                            ovl2_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
         size_t bytes = 0;

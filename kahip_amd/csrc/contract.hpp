@@ -42,6 +42,9 @@ private:
     DevBuf<long long> hcap_, hoff_;
     DevBuf<int> gchunk_prefix_, mdeg_, mpre_, ofd_;
     DevBuf<int> hk_, hv_, hm_;
+    DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, growcnt_, gbig_, gnbig_;
+    DevBuf<long long> goffm_base_, gct_base_;
+    DevBuf<int4> gpart_, gbuck_;
     DevBuf<unsigned> okey_, okey2_;
     DevBuf<unsigned long long> oval_, oval2_;
 };

@@ -35,14 +35,15 @@ constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k 
 constexpr int kBlock = 256;
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
-constexpr int kHubThreads = 512;
+constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
+constexpr int kHubDenseThreads = 512;  // refinement hub chunks (8 edges per thread)
 constexpr int kHubDense = 4096;   // dense LDS counters per hub (refinement, k <= 4096)
-constexpr int kHubDenseSlots = 16384;  // LDS counters: min(16, slots / k) private copies
+constexpr int kHubDenseSlots = 12288;  // dynamic LDS counters (48 KB): min(8, slots / k) copies
 constexpr int kFinThreads = 1024;  // hub_bucket_merge workgroup
 constexpr int kFinCap = 16384;     // LDS hash slots of hub_bucket_merge (128 KB)
 constexpr int kFinLds = 8192;      // buckets up to this many entries merge in LDS
 constexpr int kMaxBuckets = 2048;  // buckets per hub: min(kBucketsPerChunk x chunks, kMaxBuckets)
-constexpr int kBucketsPerChunk = 1;  // one bucket per chunk: fewer, larger merge tasks measured faster than 4
+constexpr int kBucketsPerChunk = 4;  // ~1000 entries per (hub, bucket): the small merge kernel takes most
 constexpr int kSmallThreads = 512;   // small bucket merge: <= kSmallMax entries in 32 KB of LDS
 constexpr int kSmallCap = 4096;
 constexpr int kSmallMax = 2048;
@@ -93,11 +94,22 @@ __device__ inline int reduce_max(int x) {
     return x;
 }
 
-// The proposal rule and the scratch updates of one proposing node. Called by one lane.
+// A proposal's contribution to the admission scratch: weight and priority into the target label
+// (inc / top) or, for a departure from an overloaded block, out of the own block (out / dtop).
+struct Contrib {
+    int label;  // -1: none
+    int w;
+    unsigned long long key;
+    bool departure;
+};
+
+// The proposal rule of one proposing node (called by one lane): writes the node's proposal slot
+// and returns the scratch contribution, which the caller adds with atomics.
 template <LpMode M>
-__device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_own,
-                              const Cand& c) {
-    if (c.conn < 0) return;  // no eligible candidate
+__device__ inline Contrib propose(const LpArgs& a, int v, int own, int w, int conn_own,
+                                  const Cand& c) {
+    Contrib r{-1, 0, 0ull, false};
+    if (c.conn < 0) return r;  // no eligible candidate
     const int size_own = a.size[own];
     bool go;
     bool overloaded = false;
@@ -108,19 +120,31 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
         go = c.conn > conn_own || overloaded ||
              (c.conn == conn_own && (long long)c.size + 2LL * w <= (long long)size_own);
     }
-    if (!go) return;
+    if (!go) return r;
     const int gain = c.conn - conn_own;
     const unsigned long long key = prio_key(gain, v);
     a.prop_t[v] = c.lab;
     a.prop_gain[v] = gain;
     a.prop_ok[v] = 1;
-    if (M == LpMode::Refine && overloaded) {
-        atomicAdd(&a.out[own], w);
-        atomicMax(&a.dtop[own], key);
+    const bool dep = M == LpMode::Refine && overloaded;
+    return Contrib{dep ? own : c.lab, w, key, dep};
+}
+
+__device__ inline void add_contrib(const LpArgs& a, const Contrib& r) {
+    if (r.label < 0) return;
+    if (r.departure) {
+        atomicAdd(&a.out[r.label], r.w);
+        atomicMax(&a.dtop[r.label], r.key);
     } else {
-        atomicAdd(&a.inc[c.lab], w);
-        atomicMax(&a.top[c.lab], key);
+        atomicAdd(&a.inc[r.label], r.w);
+        atomicMax(&a.top[r.label], r.key);
     }
+}
+
+template <LpMode M>
+__device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_own,
+                              const Cand& c) {
+    add_contrib(a, propose<M>(a, v, own, w, conn_own, c));
 }
 
 // ---------------------------------------------------------------- G lanes per node (deg <= G)
@@ -167,7 +191,44 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
         }
     }
     reduce_cand<G>(c);
-    if (valid && gl == 0) decide<M>(a, v, own, nw, conn_own, c);
+    // The block's proposals are combined per (target, kind) in LDS before the atomics: many
+    // low-degree nodes propose the same (hub) label, and one atomic per distinct target keeps
+    // them from serialising on it.
+    constexpr int NB = kBlock / G;
+    __shared__ int s_lab[NB];
+    __shared__ int s_w[NB];
+    __shared__ unsigned long long s_key[NB];
+    __shared__ uint8_t s_dep[NB];
+    const int slot = threadIdx.x / G;
+    if (gl == 0) {
+        const Contrib r = valid ? propose<M>(a, v, own, nw, conn_own, c)
+                                : Contrib{-1, 0, 0ull, false};
+        s_lab[slot] = r.label;
+        s_w[slot] = r.w;
+        s_key[slot] = r.key;
+        s_dep[slot] = r.departure ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < NB && s_lab[threadIdx.x] >= 0) {
+        const int i = threadIdx.x;
+        const int l = s_lab[i];
+        const uint8_t d = s_dep[i];
+        bool first = true;
+        for (int j = 0; j < i; ++j)
+            if (s_lab[j] == l && s_dep[j] == d) {
+                first = false;
+                break;
+            }
+        if (first) {
+            Contrib sum{l, 0, 0ull, d != 0};
+            for (int j = i; j < NB; ++j)
+                if (s_lab[j] == l && s_dep[j] == d) {
+                    sum.w += s_w[j];
+                    sum.key = s_key[j] > sum.key ? s_key[j] : sum.key;
+                }
+            add_contrib(a, sum);
+        }
+    }
     if (M == LpMode::Refine) {
         __shared__ unsigned long long tsum;
         if (threadIdx.x == 0) tsum = 0;
@@ -294,10 +355,10 @@ __device__ inline void block_reduce(int& conn_own, Cand& c) {
 // Chunked variant for refinement hubs of more than kHubChunk edges: one workgroup per chunk sums
 // its edges into per-wave LDS counters and adds the non-zero block sums to the hub's global
 // counters (dense_off[hub] + b); hub_dense_decide then decides. Single-chunk hubs decide here.
-__global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
+__global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
     int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt) {
-    __shared__ int cnt[kHubDenseSlots];
+    extern __shared__ int cnt[];  // copies x num_labels counters (dynamic: small k, more blocks)
     __shared__ int hub;
     const int chunk = blockIdx.x;
     if (threadIdx.x == 0) {
@@ -318,15 +379,15 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
     const int end = a.xadj[v + 1];
     const int e0 = beg + c_idx * kHubChunk;
     const int e1 = min(end, e0 + kHubChunk);
-    const int copies = min(kHubThreads / 64, kHubDenseSlots / num_labels);
+    const int copies = min(kHubDenseThreads / 64, kHubDenseSlots / num_labels);
     int* mine = cnt + ((threadIdx.x >> 6) % copies) * num_labels;
-    for (int i = threadIdx.x; i < copies * num_labels; i += kHubThreads) cnt[i] = 0;
+    for (int i = threadIdx.x; i < copies * num_labels; i += kHubDenseThreads) cnt[i] = 0;
     __syncthreads();
-    constexpr int EPT = kHubChunk / kHubThreads;
+    constexpr int EPT = kHubChunk / kHubDenseThreads;
     int labs[EPT], ws[EPT];
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
-        const int e = e0 + threadIdx.x + kHubThreads * j;
+        const int e = e0 + threadIdx.x + kHubDenseThreads * j;
         labs[j] = e < e1 ? a.adjncy[e] : -1;
         ws[j] = e < e1 ? a.adjwgt[e] : 0;
     }
@@ -339,7 +400,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
     __syncthreads();
     if (nchunks > 1) {
         int* out = dcnt + dense_off[gi];
-        for (int b = threadIdx.x; b < num_labels; b += kHubThreads) {
+        for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
             int conn = 0;
             for (int j = 0; j < copies; ++j) conn += cnt[j * num_labels + b];
             if (conn) atomicAdd(&out[b], conn);
@@ -350,7 +411,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
     const int nw = a.vwgt[v];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (int b = threadIdx.x; b < num_labels; b += kHubThreads) {
+    for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
         int conn = 0;
         for (int j = 0; j < copies; ++j) conn += cnt[j * num_labels + b];
         if (conn == 0) continue;
@@ -363,7 +424,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_dense_chunk(
         const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
         if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
     }
-    block_reduce<kHubThreads>(conn_own, c);
+    block_reduce<kHubDenseThreads>(conn_own, c);
     if (threadIdx.x == 0) {
         decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
         atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(end - beg));
@@ -1257,7 +1318,9 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         if (!cl && dense_hub_) {
                             const int* cp = chunk_prefix_.p + chunk_base_[sr];
                             const int* dof = dense_off_.p + hub_base_[sr];
-                            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p);
+                            const int copies = std::min(kHubDenseThreads / 64, kHubDenseSlots / nl);
+                            const size_t lds = sizeof(int) * (size_t)copies * nl;
+                            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p);
                             ps.close();
                             if (hub_multi_n_[sr] > 0) {
                                 ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_multi_n_[sr], s);

@@ -118,6 +118,34 @@ def test_contract_wide_rows(G):
         assert np.array_equal(g_, w_)
 
 
+@pytest.mark.parametrize("case", ["mod3", "hub_groups", "big_hub_cluster", "many_buckets"])
+def test_contract_chunked_tier(case):
+    """Rows with more than 32768 member edges go straight to the chunked, bucketed tier; rows
+    with many distinct neighbours fall into it after the LDS tiers."""
+    if case in ("mod3", "hub_groups", "big_hub_cluster"):
+        x, a, vw, ew = K.gen_rmat(16, 16 << 16, 11)
+    else:
+        x, a, vw, ew = K.gen_rmat(18, 16 << 18, 12)
+    n = len(x) - 1
+    if case == "mod3":
+        cl = (np.arange(n) % 3).astype(np.int32)  # three huge rows, 3 distinct neighbours each
+    elif case == "hub_groups":
+        cl = (np.arange(n) >> 4).astype(np.int32)  # 16-node groups
+    elif case == "big_hub_cluster":
+        deg = np.diff(x)
+        cl = np.arange(n, dtype=np.int32)
+        cl[np.argsort(-deg, kind="stable")[:64]] = int(np.argmax(deg))  # top hubs together
+    else:
+        rng = np.random.default_rng(5)
+        cl = np.arange(n, dtype=np.int32)
+        cl[rng.random(n) < 0.7] = 0  # one cluster of 70 % of the nodes: > 1024 chunks, capped buckets
+    cl, nc = O.remap_first_appearance(cl)
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+
+
 REFINE_CASES = [("er200w", 2, 0), ("rgg15", 2, 1), ("rgg15", 64, 2), ("delaunay15", 8, 3),
                 ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6), ("star10k", 4, 7),
                 ("star10k", 5000, 8), ("rmat12", 3000, 9)]
