@@ -128,8 +128,8 @@ int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int
  * the LP path). features: xadj[n] x 8 floats per directed edge (u, v):
  *   0 log2(1 + min(deg u, deg v))  1 log2(1 + max deg)  2 log2(1 + w(e))
  *   3 log2(1 + min(c(u), c(v)))    4 log2(1 + max c)    5 popcount(sig(u) & sig(v)) / 8
- *   6 number of 3 one-round LP clusterings (cluster bound `bound`, seeds hash3(seed, 0xA6, i))
- *     that put u and v together
+ *   6 number of 3 one-round, one-sub-round LP clusterings (cluster bound `bound`, seeds
+ *     hash3(seed, 0xA6, i)) that put u and v together
  *   7 log2(1 + w(e)^2 / max(c(u) c(v), 1))
  * with sig(x) = OR over neighbours y of 1 << (mix32(y) & 63). */
 int kahip_edge_features(int n, const int* xadj, const int* adjncy, const int* vwgt,

@@ -6,7 +6,7 @@
 // Features of a directed edge e = (u, v), symmetric in u and v, from integers only:
 //   f0 log2(1 + min deg)   f1 log2(1 + max deg)    f2 log2(1 + w(e))
 //   f3 log2(1 + min c)     f4 log2(1 + max c)      f5 popcount(sig(u) & sig(v)) / 8
-//   f6 number of 3 seeded one-round LP sweeps that put u and v in the same cluster
+//   f6 number of 3 seeded one-round, one-sub-round LP sweeps that put u and v in the same cluster
 //   f7 log2(1 + expansion*^2)
 // sig(x) is a 64-bit sketch of x's neighbourhood (bit mix32(y) & 63 set for every neighbour y),
 // so f5 is a capped, collision-prone common-neighbour count computed in O(1) per edge.
@@ -60,11 +60,14 @@ __global__ void sketch_kernel(int n, const int* __restrict__ xadj, const int* __
 
 __device__ inline float lg(float x) { return __log2f(1.0f + x); }
 
+// OUT = float: 8 floats per edge (the features API); OUT = bf16x8: the 8 features rounded to
+// BF16 (round to nearest even, as the MLP rounds its inputs), 16 B per edge.
+template <class OUT>
 __global__ __launch_bounds__(256) void features_kernel(
     int n, long long m2, const int* __restrict__ xadj, const int* __restrict__ adjncy,
     const int* __restrict__ adjwgt, const int* __restrict__ vwgt,
     const unsigned long long* __restrict__ sig, const int* __restrict__ l0,
-    const int* __restrict__ l1, const int* __restrict__ l2, float* __restrict__ out) {
+    const int* __restrict__ l1, const int* __restrict__ l2, OUT* __restrict__ out) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long e0 = t * 8;
     if (e0 >= m2) return;
@@ -88,15 +91,18 @@ __global__ __launch_bounds__(256) void features_kernel(
         const float cn = (float)__popcll(sig[u] & sig[v]);
         const float agree = (float)((l0[u] == l0[v]) + (l1[u] == l1[v]) + (l2[u] == l2[v]));
         const float es2 = (w * w) / fmaxf(cu * cv, 1.0f);
-        float* o = out + e * kFeatures;
-        o[0] = lg(fminf(du, dv));
-        o[1] = lg(fmaxf(du, dv));
-        o[2] = lg(w);
-        o[3] = lg(fminf(cu, cv));
-        o[4] = lg(fmaxf(cu, cv));
-        o[5] = cn * 0.125f;
-        o[6] = agree;
-        o[7] = lg(es2);
+        const float f[kFeatures] = {lg(fminf(du, dv)), lg(fmaxf(du, dv)), lg(w),
+                                    lg(fminf(cu, cv)), lg(fmaxf(cu, cv)), cn * 0.125f, agree,
+                                    lg(es2)};
+        if constexpr (sizeof(OUT) == sizeof(float)) {
+#pragma unroll
+            for (int j = 0; j < kFeatures; ++j) out[e * kFeatures + j] = f[j];
+        } else {
+            bf16x8 b;
+#pragma unroll
+            for (int j = 0; j < kFeatures; ++j) b[j] = (__bf16)f[j];
+            out[e] = b;
+        }
     }
 }
 
@@ -169,10 +175,13 @@ __host__ __device__ inline int hid_of(int kb, int g, int j) {
     return 16 * (2 * kb + (j >= 4 ? 1 : 0)) + 4 * g + (j & 3);
 }
 
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ feat,
+// logit != nullptr: writes the logits; otherwise writes score = quantize(logit) >> shift, >= 1.
+template <class IN>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const IN* __restrict__ feat,
                                                           long long m,
                                                           const uint16_t* __restrict__ pack,
-                                                          float* __restrict__ logit) {
+                                                          float* __restrict__ logit,
+                                                          int* __restrict__ score, int shift) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
     const bf16x8* A = reinterpret_cast<const bf16x8*>(pack);
@@ -199,14 +208,14 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
     for (long long base = wave * 16; base < m; base += waves * 16) {
         const long long e = base + col;
         // B operand of layer 1: the 8 features of edge `col` in lanes of group 0
-        bf16x8 xb;
+        bf16x8 xb = {};
+        if (g == 0 && e < m) {
+            if constexpr (sizeof(IN) == sizeof(float)) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float f = (g == 0 && e < m) ? feat[e * kFeatures + j] : 0.0f;
-            const uint16_t bits = f2bf(f);
-            __bf16 h;
-            memcpy(&h, &bits, 2);
-            xb[j] = h;
+                for (int j = 0; j < 8; ++j) xb[j] = (__bf16)feat[e * kFeatures + j];
+            } else {
+                xb = feat[e];
+            }
         }
         f32x4 h1[4];
 #pragma unroll
@@ -223,12 +232,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
         for (int kb = 0; kb < 2; ++kb) {
             bf16x8 hb;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint16_t bits = f2bf(h1[2 * kb + (j >> 2)][j & 3]);
-                __bf16 h;
-                memcpy(&h, &bits, 2);
-                hb[j] = h;
-            }
+            for (int j = 0; j < 8; ++j) hb[j] = (__bf16)h1[2 * kb + (j >> 2)][j & 3];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 h2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t][kb], hb, h2[t], 0, 0, 0);
@@ -240,7 +244,10 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
             for (int r = 0; r < 4; ++r) acc += w3[t][r] * fmaxf(h2[t][r] + b2[t][r], 0.0f);
         acc += __shfl_xor(acc, 16, 64);
         acc += __shfl_xor(acc, 32, 64);
-        if (g == 0 && e < m) logit[e] = acc + b3;
+        if (g == 0 && e < m) {
+            if (logit) logit[e] = acc + b3;
+            else score[e] = max(quantize_logit(acc + b3) >> shift, 1);
+        }
     }
 }
 
@@ -423,45 +430,56 @@ void pack_weights(const MlpWeights& w, std::vector<uint8_t>& out) {
 
 }  // namespace
 
-void RatingWorkspace::features(const DGraph& g, int bound, uint32_t seed, float* out,
-                               hipStream_t s, LpWorkspace& lp, Profiler* prof) {
-    ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
+void RatingWorkspace::sweeps_and_sketch(const DGraph& g, int bound, uint32_t seed, hipStream_t s,
+                                        LpWorkspace& lp) {
     sig_.reserve(g.n);
     lab_.reserve(3LL * g.n);
     size_.reserve(g.n);
     if (g.n > 0)
         hipLaunchKernelGGL(sketch_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
                            g.adjncy, sig_.p);
+    // three one-round sweeps with a single sub-round: the node lists do not depend on the seed
+    // then, so they are built once; the seeds only change the tie-breaking
     BinConfig bins;
+    lp.prepare(LpMode::Cluster, g, g.n, 1, 0u, bins, s);
     for (int i = 0; i < 3; ++i) {
         int* lab = lab_.p + (long long)i * g.n;
         const uint32_t sd = hash3(seed, 0xA6u, (uint32_t)i);
         LpWorkspace::init_cluster_labels(g, lab, size_.p, s);
-        lp.prepare(LpMode::Cluster, g, g.n, 4, sd, bins, s);
         lp.run(LpMode::Cluster, g, lab, size_.p, bound, 1, sd, s, nullptr, "rating_sweep");
     }
+}
+
+void RatingWorkspace::features(const DGraph& g, int bound, uint32_t seed, float* out,
+                               hipStream_t s, LpWorkspace& lp, Profiler* prof) {
+    ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
+    sweeps_and_sketch(g, bound, seed, s, lp);
     if (g.m2 > 0)
-        hipLaunchKernelGGL(features_kernel, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256), 0, s,
-                           g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p, lab_.p,
+        hipLaunchKernelGGL(features_kernel<float>, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256),
+                           0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p, lab_.p,
                            lab_.p + g.n, lab_.p + 2LL * g.n, out);
     KH_CHECK(hipGetLastError());
 }
 
-void RatingWorkspace::mlp_forward(const float* feat, long long m, const MlpWeights& w,
-                                  float* logits, hipStream_t s, Profiler* prof) {
+void RatingWorkspace::upload_weights(const MlpWeights& w, hipStream_t s) {
     std::vector<uint8_t> host;
     pack_weights(w, host);
     wpack_.reserve(kPackBytes / 2);
     KH_CHECK(hipMemcpyAsync(wpack_.p, host.data(), kPackBytes, hipMemcpyHostToDevice, s));
+    KH_CHECK(hipStreamSynchronize(s));  // `host` must outlive the copy
+}
+
+void RatingWorkspace::mlp_forward(const float* feat, long long m, const MlpWeights& w,
+                                  float* logits, hipStream_t s, Profiler* prof) {
+    upload_weights(w, s);
     if (m > 0) {
         // 9344 useful FLOP per edge (2 x (8*64 + 64*64 + 64))
         ProfScope ps(prof, "mlp_forward", 9344.0 * (double)m, m, s);
         const long long groups = (m + 15) / 16;
         const int blocks = (int)std::min<long long>((groups + 3) / 4, 8192);
-        hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(256), 0, s, feat, m, wpack_.p,
-                           logits);
+        hipLaunchKernelGGL(mlp_forward_kernel<float>, dim3(blocks), dim3(256), 0, s, feat, m,
+                           wpack_.p, logits, nullptr, 0);
     }
-    KH_CHECK(hipStreamSynchronize(s));  // `host` must outlive the copy
     KH_CHECK(hipGetLastError());
 }
 
@@ -493,12 +511,25 @@ void RatingWorkspace::scores(RatingMode mode, const DGraph& g, int bound, uint32
         return;
     }
     KH_REQUIRE(mode == RatingMode::Learned && w, "learned rating needs MLP weights");
-    feat_.reserve(g.m2 * kFeatures);
-    logit_.reserve(g.m2);
-    features(g, bound, seed, feat_.p, s, lp, prof);
-    mlp_forward(feat_.p, g.m2, *w, logit_.p, s, prof);
-    hipLaunchKernelGGL(quantize_kernel, dim3(grid_for(g.m2, 256)), dim3(256), 0, s, g.m2, logit_.p,
-                       shift, score);
+    // features rounded to BF16 (16 B per edge), then the MLP writes the quantised scores
+    featb_.reserve(g.m2 * 8);
+    upload_weights(*w, s);
+    {
+        ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
+        sweeps_and_sketch(g, bound, seed, s, lp);
+        hipLaunchKernelGGL(features_kernel<bf16x8>, dim3(grid_for((g.m2 + 7) / 8, 256)),
+                           dim3(256), 0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p,
+                           lab_.p, lab_.p + g.n, lab_.p + 2LL * g.n,
+                           reinterpret_cast<bf16x8*>(featb_.p));
+    }
+    {
+        ProfScope ps(prof, "mlp_forward", 9344.0 * (double)g.m2, g.m2, s);
+        const long long groups = (g.m2 + 15) / 16;
+        const int blocks = (int)std::min<long long>((groups + 3) / 4, 8192);
+        hipLaunchKernelGGL(mlp_forward_kernel<bf16x8>, dim3(blocks), dim3(256), 0, s,
+                           reinterpret_cast<const bf16x8*>(featb_.p), g.m2, wpack_.p, nullptr,
+                           score, shift);
+    }
     KH_CHECK(hipGetLastError());
 }
 

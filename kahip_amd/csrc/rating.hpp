@@ -53,10 +53,13 @@ public:
     int score_shift(const DGraph& g, hipStream_t s);
 
 private:
+    void sweeps_and_sketch(const DGraph& g, int bound, uint32_t seed, hipStream_t s,
+                           LpWorkspace& lp);
+    void upload_weights(const MlpWeights& w, hipStream_t s);
     DevBuf<unsigned long long> sig_;
     DevBuf<int> lab_, size_, map_;
     DevBuf<float> feat_, logit_, partial_, loss_;
-    DevBuf<uint16_t> wpack_;
+    DevBuf<uint16_t> wpack_, featb_;
 };
 
 // score = clamp(round(logit * 128) + 2048, 1, kScoreMax); the same formula on the host.

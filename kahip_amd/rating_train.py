@@ -5,8 +5,8 @@ the integers 16..20 per graph), alternating kinds, generator seeds 1000 + i (dis
 seeds 0..99 used for evaluation). Each graph is partitioned by this library's own plain
 pipeline (k = 64, eps = 3%); an edge is labelled 1 if its endpoints share a block (uncut) and 0
 otherwise. Features are those of kahip_edge_features on the input graph with the k = 64 cluster
-bound. A fixed number of edges per graph is sampled (numpy Generator, seed 0), all examples are
-shuffled once, and the MLP is trained by Adam with the library's GPU forward/backward kernels
+bound. Per graph an equal number of cut and uncut edges is sampled (numpy Generator, seed 0; at
+most per_graph / 2 of each), all examples are shuffled once, and the MLP is trained by Adam with the library's GPU forward/backward kernels
 (kahip_mlp_train). Everything is seeded, so a rerun writes the same weights.
 
     python -m kahip_amd.rating_train [--graphs 200] [--per-graph 40000] [--epochs 6]
@@ -69,7 +69,12 @@ def collect(num_graphs, per_graph, k=64, log=print):
         f = K.graph_edge_features(g, cluster_bound(inf["n"], k), 7919 * i + 1)
         src = np.repeat(np.arange(inf["n"]), np.diff(x))
         y = (part[src] == part[a]).astype(np.float32)
-        take = rng.choice(len(y), size=min(per_graph, len(y)), replace=False)
+        # class-balanced sample: the score only ranks the edges around one node, so the model
+        # should separate cut from uncut edges within a graph, not one graph from another
+        pos, neg = np.nonzero(y > 0.5)[0], np.nonzero(y < 0.5)[0]
+        half = min(per_graph // 2, len(pos), len(neg))
+        take = np.concatenate([rng.choice(pos, size=half, replace=False),
+                               rng.choice(neg, size=half, replace=False)])
         feats.append(f[take])
         labels.append(y[take])
         meta.append({"kind": kind, "log2n": lg, "n": inf["n"], "m2": inf["m2"],

@@ -62,12 +62,12 @@ def _popcount64(x):
 
 
 def sweep_labels(xadj, adjncy, vwgt, adjwgt, bound, seed):
-    """The three one-round synchronous LP clusterings behind feature 6 (4 sub-rounds each)."""
+    """The three one-round synchronous LP clusterings behind feature 6 (one sub-round each)."""
     out = []
     for i in range(3):
         sd = hash3(seed, 0xA6, i)
         lab, _, _ = lp_cluster_sync(np.asarray(xadj, np.int32), np.asarray(adjncy, np.int32),
-                                    vwgt, adjwgt, bound, 1, 4, sd)
+                                    vwgt, adjwgt, bound, 1, 1, sd)
         out.append(np.asarray(lab))
     return out
 
