@@ -258,6 +258,19 @@ def main():
     lp_gbs = (sum(v["bytes"] for v in lp_all) / (sum(v["ms"] for v in lp_all) / 1e3) / 1e9
               if lp_all else None)
 
+    # the learned edge rating on the same graph (untimed; its GPU time is reported beside)
+    learned = None
+    try:
+        wts = K.load_rating_weights()
+        _, lst = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False,
+                             rating=K.RATING_LEARNED, mlp_weights=wts)
+        learned = {"cut": lst["cut"], "balance": round(lst["balance"], 5),
+                   "gpu_ms": round(lst["ms_coarsen"] + lst["ms_refine"] + lst["ms_cut"], 2),
+                   "rating_ms": round(lst["ms_rating"], 2)}
+        log(f"learned rating: cut={lst['cut']} (plain {stats[-1]['cut']}) gpu_ms={learned['gpu_ms']}")
+    except Exception as e:  # reported, never silently replaced
+        log(f"learned rating run failed: {e!r}")
+
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
         try:
@@ -294,7 +307,8 @@ def main():
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 4),
         "lp_cluster_propose_gbs": None if lp_gbs is None else round(lp_gbs, 1),
         "edge_cut_plain": last["cut"],
-        "edge_cut_learned": None,
+        "edge_cut_learned": None if learned is None else learned["cut"],
+        "learned_rating": learned,
         "balance": round(last["balance"], 5),
         "levels": last["level_n"],
         "roofline": roofline,

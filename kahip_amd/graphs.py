@@ -5,6 +5,9 @@ Formats follow the reference's readers/writers:
     line per node, 1-based neighbour ids, optional node weight first (fmt 10/11) and an edge weight
     after each neighbour (fmt 1/11); lines starting with '%' are comments.
   * partition file: one block id per line (lib/io/graph_io.cpp:50-80, 194-203).
+  * BGF binary graph (parallel/parallel_src/lib/io/parallel_graph_io.cpp:474-523): little-endian
+    uint64 words: version 3, n, number of directed entries; n + 1 byte offsets of the rows
+    (the first is the header size, (3 + n + 1) * 8); the neighbour ids.
 
 Graphs are returned as ``(xadj, adjncy, vwgt, adjwgt)`` int32 numpy arrays (0-based CSR, both
 directions stored); vwgt / adjwgt are None for unit weights.
@@ -30,6 +33,42 @@ import numpy as np
 M64 = (1 << 64) - 1
 K1 = 0xD1B54A32D192ED03
 K2 = 0x8CB92BA72F3D8DD7
+
+
+# ----------------------------------------------------------------------------- BGF binary
+BGF_VERSION = 3
+
+
+def write_bgf(path, xadj, adjncy):
+    """Write the unweighted structure of a graph as BGF."""
+    xadj = np.asarray(xadj, np.uint64)
+    n = len(xadj) - 1
+    m2 = int(xadj[-1])
+    head = np.array([BGF_VERSION, n, m2], np.uint64)
+    offs = np.uint64((3 + n + 1) * 8) + xadj * np.uint64(8)
+    with open(path, "wb") as fh:
+        head.astype("<u8").tofile(fh)
+        offs.astype("<u8").tofile(fh)
+        np.asarray(adjncy, np.uint64).astype("<u8").tofile(fh)
+
+
+def read_bgf(path):
+    """Read a BGF graph; returns (xadj, adjncy, None, None) as int32."""
+    head = np.fromfile(path, dtype="<u8", count=3)
+    if len(head) != 3 or int(head[0]) != BGF_VERSION:
+        raise ValueError(f"{path}: not a BGF file of version {BGF_VERSION}")
+    n, m2 = int(head[1]), int(head[2])
+    words = np.fromfile(path, dtype="<u8", count=3 + n + 1 + m2)
+    if len(words) != 3 + n + 1 + m2:
+        raise ValueError(f"{path}: truncated")
+    offs = words[3:3 + n + 1].astype(np.int64)
+    if offs[0] != (3 + n + 1) * 8 or np.any(np.diff(offs) < 0) or offs[-1] != offs[0] + 8 * m2:
+        raise ValueError(f"{path}: inconsistent offsets")
+    xadj = ((offs - offs[0]) // 8).astype(np.int32)
+    adjncy = words[3 + n + 1:].astype(np.int64)
+    if m2 and (adjncy.max() >= n):
+        raise ValueError(f"{path}: neighbour id out of range")
+    return xadj, adjncy.astype(np.int32), None, None
 
 
 # ----------------------------------------------------------------------------- METIS IO
