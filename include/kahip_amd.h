@@ -35,6 +35,8 @@ typedef struct {
     int reorder;         /* renumber the input by degree first: -1 automatic (maximum degree
                             above 32 x the average), 0 off, 1 on; the partition is returned in
                             the caller's numbering */
+    int ensemble;        /* LP clusterings per level, overlaid (1 = single clustering; the
+                            STRONGSOCIAL preset uses 18/17/15/7/3 for k <= 3/7/15/31/more) */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);

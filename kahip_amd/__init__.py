@@ -46,7 +46,8 @@ class Params(C.Structure):
                 ("cluster_factor", C.c_int), ("stop_factor", C.c_int),
                 ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
                 ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
-                ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int)]
+                ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int),
+                ("ensemble", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -217,6 +218,11 @@ def default_params(**kw) -> Params:
     return p
 
 
+def ensemble_size(k):
+    """LP clusterings per level of the STRONGSOCIAL preset (app/configuration.h:494-509)."""
+    return 18 if k <= 3 else 17 if k <= 7 else 15 if k <= 15 else 7 if k <= 31 else 3
+
+
 # ----------------------------------------------------------------------------- drop-in API
 def kaffpa(vwgt, xadj, adjcwgt, adjncy, nparts, imbalance, suppress_output, seed, mode):
     """Same arguments and return value as the reference's ``kahip.kaffpa``: ``(edgecut, blocks)``.
@@ -229,7 +235,8 @@ def kaffpa(vwgt, xadj, adjcwgt, adjncy, nparts, imbalance, suppress_output, seed
     ew = None if adjcwgt is None or len(adjcwgt) == 0 else _i32(adjcwgt)
     part = np.zeros(max(n, 1), np.int32)
     cut = C.c_int(0)
-    p = default_params(k=int(nparts), imbalance=float(imbalance), seed=int(seed))
+    p = default_params(k=int(nparts), imbalance=float(imbalance), seed=int(seed),
+                       ensemble=ensemble_size(int(nparts)) if mode == STRONGSOCIAL else 1)
     _keep = (vw, ew)
     _check(lib().kaffpa_gpu_ex(n, _p(vw), xadj, _p(ew), adjncy, C.byref(p), C.byref(cut), part,
                                None))

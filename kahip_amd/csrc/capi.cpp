@@ -69,6 +69,7 @@ PartitionParams to_params(const kahip_params* q) {
     p.rating_levels = q->rating_levels;
     p.mlp = reinterpret_cast<const MlpWeights*>(q->mlp_weights);
     p.reorder = q->reorder;
+    p.ensemble = q->ensemble;
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
@@ -239,6 +240,7 @@ void kahip_default_params(kahip_params* p) {
     p->rating_levels = d.rating_levels;
     p->mlp_weights = nullptr;
     p->reorder = d.reorder;
+    p->ensemble = d.ensemble;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }
@@ -252,12 +254,14 @@ void kaffpa(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts
             double* imbalance, bool suppress_output, int seed, int mode, int* edgecut,
             int* part) {
     (void)suppress_output;  // ignored by the reference too (kaHIP_interface.cpp:108-113)
-    (void)mode;
     kahip_params p;
     kahip_default_params(&p);
     p.k = *nparts;
     p.imbalance = *imbalance;
     p.seed = seed;
+    // STRONGSOCIAL clusters with ensembles (app/configuration.h:584-591); its FM refinement is
+    // not part of this library, which refines with LP in every mode
+    if (mode == STRONGSOCIAL) p.ensemble = ensemble_size_for(p.k);
     if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
         report("kaffpa", edgecut);
 }

@@ -12,6 +12,8 @@
 
 #include "reorder.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -25,6 +27,26 @@ inline int grid_for(long long work, int block) { return (int)((work + block - 1)
 __global__ void fill_ones(int* p, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 1;
+}
+
+// keys[v] = (a[v] << 32) | b[v], ids[v] = v
+__global__ void pair_keys(int n, const int* __restrict__ a, const int* __restrict__ b,
+                          unsigned long long* __restrict__ keys, int* __restrict__ ids) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    keys[v] = ((unsigned long long)(uint32_t)a[v] << 32) | (uint32_t)b[v];
+    ids[v] = v;
+}
+
+// After a stable sort by pair key: every node gets the smallest node id of its pair group (the
+// first element of its run).
+__global__ void run_leaders(int n, const unsigned long long* __restrict__ skeys,
+                            const int* __restrict__ sids, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i > 0 && skeys[i] == skeys[i - 1]) return;
+    const int lead = sids[i];
+    for (int j = i; j < n && skeys[j] == skeys[i]; ++j) out[sids[j]] = lead;
 }
 
 struct EventPair {
@@ -45,6 +67,14 @@ struct EventPair {
 };
 
 }  // namespace
+
+int ensemble_size_for(int k) {
+    if (k <= 3) return 18;
+    if (k <= 7) return 17;
+    if (k <= 15) return 15;
+    if (k <= 31) return 7;
+    return 3;
+}
 
 unsigned upper_bound_for(long long total_weight, int k, double imbalance) {
     const double imb_percent = 100 * imbalance;
@@ -123,6 +153,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     KH_REQUIRE(p.imbalance >= 0, "imbalance must be >= 0");
     KH_REQUIRE(input.total_vwgt < (1LL << 31), "total node weight must be < 2^31");
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs MLP weights");
+    KH_REQUIRE(p.ensemble >= 1 && p.ensemble <= 64, "ensemble must be in [1, 64]");
     PartitionStats local;
     PartitionStats& S = st ? *st : local;
     S = PartitionStats();
@@ -151,7 +182,28 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     std::vector<std::unique_ptr<OwnedGraph>> coarse;  // coarse[l] = level l+1
     std::vector<DevBuf<int>> maps;                     // maps[l]: level l -> level l+1
     auto level_graph = [&](int l) -> const DGraph& { return l == 0 ? g0 : coarse[l - 1]->g; };
-    DevBuf<int> label, size, score;
+    DevBuf<int> label, size, score, elabel;
+    // label := the smallest node id among the nodes sharing (label, other)
+    DevBuf<unsigned long long> okeys, okeys2;
+    DevBuf<int> oids, oids2;
+    DevBuf<char> otmp;
+    auto overlay = [&](int n, int* lab, const int* other, hipStream_t st) {
+        okeys.reserve(n);
+        okeys2.reserve(n);
+        oids.reserve(n);
+        oids2.reserve(n);
+        hipLaunchKernelGGL(pair_keys, dim3(grid_for(n, 256)), dim3(256), 0, st, n, lab, other,
+                           okeys.p, oids.p);
+        size_t bytes = 0;
+        KH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, okeys.p, okeys2.p, oids.p, oids2.p,
+                                           (size_t)n, 0, 64, st));
+        otmp.reserve((long long)bytes + 1);
+        KH_CHECK(rocprim::radix_sort_pairs(otmp.p, bytes, okeys.p, okeys2.p, oids.p, oids2.p,
+                                           (size_t)n, 0, 64, st));
+        hipLaunchKernelGGL(run_leaders, dim3(grid_for(n, 256)), dim3(256), 0, st, n, okeys2.p,
+                           oids2.p, lab);
+        KH_CHECK(hipGetLastError());
+    };
     const unsigned num_stop = (unsigned)p.stop_factor * (unsigned)p.k;
     int level = 0;
     S.level_n[0] = g0.n;
@@ -183,6 +235,18 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         long long edges_round = 0;
         for (int i = 0; i < p.subrounds * kNumBins; ++i) edges_round += lp.host_edges[i];
         S.edges_cluster += edges_round * p.cluster_rounds;
+        for (int run = 1; run < p.ensemble; ++run) {
+            // another clustering with its own seed and cluster factor, overlaid on the result
+            const uint32_t eseed = ensemble_seed(p.seed, level, run);
+            const int eb = (int)cluster_bound_for(U, ensemble_factor(p.seed, level, run));
+            elabel.reserve(fine.n);
+            LpWorkspace::init_cluster_labels(fine, elabel.p, size.p, s);
+            lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, eseed, p.bins, s);
+            lp.run(LpMode::Cluster, rated, elabel.p, size.p, eb, p.cluster_rounds, eseed, s, prof,
+                   "lp_cluster");
+            S.edges_cluster += edges_round * p.cluster_rounds;
+            overlay(fine.n, label.p, elabel.p, s);
+        }
         maps.emplace_back();
         maps.back().reserve(fine.n);
         const int nc = cw.remap(fine, label.p, maps.back().p, s, prof);

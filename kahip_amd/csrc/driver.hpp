@@ -35,6 +35,10 @@ struct PartitionParams {
     // (None: the edge weights themselves, as in the FASTSOCIAL preset).
     RatingMode rating = RatingMode::None;
     int reorder = -1;  // degree renumbering of the input: -1 automatic (skewed), 0 off, 1 on
+    // Ensemble clusterings (size_constraint_label_propagation.cpp:91-123): that many LP
+    // clusterings per level, overlaid (two nodes share a coarse node iff every clustering puts
+    // them together); 1 = a single clustering (FASTSOCIAL).
+    int ensemble = 1;
     int rating_levels = kMaxLevels;
     const MlpWeights* mlp = nullptr;  // required for RatingMode::Learned
 };
@@ -60,6 +64,16 @@ inline uint32_t cluster_seed(int seed, int level) {
 }
 inline uint32_t refine_seed(int seed, int level) {
     return hash3((uint32_t)seed, 0x4Eu, (uint32_t)level);
+}
+// The reference's clusterings number for the social presets (app/configuration.h:494-509).
+int ensemble_size_for(int k);
+// Cluster factor of ensemble run i >= 1: 10 + hash % 21, i.e. in [10, 30] as the reference's
+// nextInt(10, 30) (size_constraint_label_propagation.cpp:119); run 0 uses the configured factor.
+inline int ensemble_factor(int seed, int level, int run) {
+    return 10 + (int)(hash3((uint32_t)seed, 0xE5u + (uint32_t)level, (uint32_t)run) % 21u);
+}
+inline uint32_t ensemble_seed(int seed, int level, int run) {
+    return hash3((uint32_t)seed, 0xE6u + (uint32_t)level, (uint32_t)run);
 }
 inline uint32_t rating_seed(int seed, int level) {
     return hash3((uint32_t)seed, 0x5Au, (uint32_t)level);

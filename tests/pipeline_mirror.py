@@ -52,28 +52,43 @@ def reorder_by_degree(xadj, adjncy, vwgt, adjwgt):
     return (nx.astype(np.int32), na, nv, nw), inv
 
 
+def ensemble_factor(seed, level, run):
+    return 10 + O.hash3(seed, 0xE5 + level, run) % 21
+
+
+def ensemble_seed(seed, level, run):
+    return O.hash3(seed, 0xE6 + level, run)
+
+
+def overlay(a, b):
+    """Smallest node id among the nodes sharing (a[v], b[v])."""
+    key = np.asarray(a, np.int64) * (len(a) + 1) + np.asarray(b, np.int64)
+    _, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    return first[inv].astype(np.int32)
+
+
 def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, cluster_rounds=10,
                      refine_rounds=25, subrounds=4, cluster_factor=18, stop_factor=5000,
-                     ip=None, scores=None, rating_levels=64, reorder=-1):
+                     ip=None, scores=None, rating_levels=64, reorder=-1, ensemble=1):
     """As mirror_partition_plain, after the library's degree renumbering of skewed inputs
     (reorder: -1 automatic, 0 off, 1 on); the partition is returned in the input numbering."""
     if reorder_wanted(xadj, reorder):
         g, inv = reorder_by_degree(xadj, adjncy, vwgt, adjwgt)
         part, info = mirror_partition_plain(*g, k, imbalance, seed, cluster_rounds,
                                             refine_rounds, subrounds, cluster_factor,
-                                            stop_factor, ip, scores, rating_levels)
+                                            stop_factor, ip, scores, rating_levels, ensemble)
         info["reordered"] = True
         return part[inv], info
     part, info = mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance, seed,
                                         cluster_rounds, refine_rounds, subrounds, cluster_factor,
-                                        stop_factor, ip, scores, rating_levels)
+                                        stop_factor, ip, scores, rating_levels, ensemble)
     info["reordered"] = False
     return part, info
 
 
 def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0,
                            cluster_rounds=10, refine_rounds=25, subrounds=4, cluster_factor=18,
-                           stop_factor=5000, ip=None, scores=None, rating_levels=64):
+                           stop_factor=5000, ip=None, scores=None, rating_levels=64, ensemble=1):
     """Returns (part, info). `ip(xadj, adjncy, vwgt, adjwgt, k, U, reps, seed)` partitions the
     coarsest graph (the library's kahip_amd.initial_partition by default). `scores(xadj, adjncy,
     vwgt, adjwgt, B, rating_seed)` (optional) gives the int32 edge scores that replace the edge
@@ -98,6 +113,13 @@ def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0
         if scores is not None and level < rating_levels and int(x[-1]) > 0:
             rw = np.asarray(scores(x, a, vw, ew, B, O.hash3(seed, 0x5A, level)), np.int32)
         cl, nc, _ = O.lp_cluster_sync(x, a, vw, rw, B, cluster_rounds, subrounds, cs)
+        for run in range(1, ensemble):
+            eb = O.cluster_bound(U, ensemble_factor(seed, level, run))
+            other, _, _ = O.lp_cluster_sync(x, a, vw, rw, eb, cluster_rounds, subrounds,
+                                            ensemble_seed(seed, level, run))
+            cl = overlay(cl, other)
+        if ensemble > 1:
+            cl, nc = O.remap_first_appearance(cl)
         cx, ca, cw, cv = O.contract(x, a, vw, ew, cl, nc)
         maps.append(cl)
         levels.append((cx, ca, cv, cw))

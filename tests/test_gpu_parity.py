@@ -213,6 +213,23 @@ def test_pipeline_reorder_modes_match_mirror(G, name, k, seed, reorder):
     assert st["cut"] == O.edge_cut(x, a, ew, want)
 
 
+@pytest.mark.parametrize("name,k,seed,ens", [("rgg15", 4, 1, 3), ("rmat12", 8, 2, 2),
+                                              ("weighted132", 2, 0, 4)])
+def test_pipeline_ensemble_matches_mirror(G, name, k, seed, ens):
+    x, a, vw, ew = G[name]
+    part, st = K.partition(x, a, vw, ew, k=k, imbalance=0.03, seed=seed, ensemble=ens)
+    want, info = mirror_partition(x, a, vw, ew, k, 0.03, seed, ensemble=ens)
+    assert st["level_n"] == info["level_n"]
+    assert np.array_equal(part, want)
+
+
+def test_kaffpa_strongsocial_uses_ensembles(G):
+    x, a, vw, ew = G["rgg15"]
+    cut, blocks = K.kaffpa(vw, x, ew, a, 16, 0.03, True, 3, K.STRONGSOCIAL)
+    part, st = K.partition(x, a, vw, ew, k=16, imbalance=0.03, seed=3, ensemble=7)
+    assert np.array_equal(np.array(blocks), part) and cut == st["cut"]
+
+
 def test_pipeline_deterministic(G):
     x, a, vw, ew = G["rmat12"]
     p1, s1 = K.partition(x, a, vw, ew, k=16, seed=5)
