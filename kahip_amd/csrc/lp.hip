@@ -38,6 +38,7 @@ constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
 constexpr int kHubDenseThreads = 512;  // refinement hub chunks (8 edges per thread)
 constexpr int kHubDense = 4096;   // dense LDS counters per hub (refinement, k <= 4096)
+constexpr long long kAllHubCounters = 1LL << 28;  // counters (ints) for incremental hubs
 constexpr int kHubDenseSlots = 12288;  // dynamic LDS counters (48 KB): min(8, slots / k) copies
 constexpr int kFinThreads = 1024;  // hub_bucket_merge workgroup
 constexpr int kFinCap = 16384;     // LDS hash slots of hub_bucket_merge (128 KB)
@@ -355,9 +356,14 @@ __device__ inline void block_reduce(int& conn_own, Cand& c) {
 // Chunked variant for refinement hubs of more than kHubChunk edges: one workgroup per chunk sums
 // its edges into per-wave LDS counters and adds the non-zero block sums to the hub's global
 // counters (dense_off[hub] + b); hub_dense_decide then decides. Single-chunk hubs decide here.
+// Hubs with global counters (dense_off >= 0) add their chunk sums there and are decided by
+// hub_dense_decide. build: only those hubs, regardless of the frontier (the initial counters of
+// the incremental scheme); incremental: those hubs are skipped (the moves keep their counters up
+// to date).
 __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt) {
+    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt, int build,
+    int incremental) {
     extern __shared__ int cnt[];  // copies x num_labels counters (dynamic: small k, more blocks)
     __shared__ int hub;
     const int chunk = blockIdx.x;
@@ -372,8 +378,10 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     __syncthreads();
     const int gi = hub;
     const int v = list[gi];
-    if (a.active[v] == 0) return;  // uniform per block
     const int nchunks = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const bool counted = dense_off[gi] >= 0;  // the hub keeps global counters
+    if (build ? !counted : (a.active[v] == 0 || (incremental && counted)))
+        return;  // uniform per block
     const int c_idx = chunk - chunk_prefix[gi];
     const int beg = a.xadj[v];
     const int end = a.xadj[v + 1];
@@ -398,7 +406,7 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     for (int j = 0; j < EPT; ++j)
         if (labs[j] >= 0) atomicAdd(&mine[labs[j]], ws[j]);
     __syncthreads();
-    if (nchunks > 1) {
+    if (counted) {
         int* out = dcnt + dense_off[gi];
         for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
             int conn = 0;
@@ -431,10 +439,11 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     }
 }
 
-// One workgroup per multi-chunk refinement hub: decides from the summed counters and clears them.
+// One workgroup per multi-chunk refinement hub: decides from the summed counters and clears them
+// (keep: the incremental counters stay).
 __global__ __launch_bounds__(256) void hub_dense_decide(
     LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
-    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt) {
+    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt, int keep) {
     const int mi = blockIdx.x;
     if (mi >= n_multi) return;
     const int gi = multi[mi];
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(256) void hub_dense_decide(
     for (int b = threadIdx.x; b < num_labels; b += 256) {
         const int conn = cnt[b];
         if (conn == 0) continue;
-        cnt[b] = 0;
+        if (!keep) cnt[b] = 0;
         if (b == own) {
             conn_own = conn;
             continue;
@@ -862,9 +871,21 @@ __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restri
             if (M == LpMode::Refine) {
                 const int beg = a.xadj[v], end = a.xadj[v + 1];
                 if (end - beg > kBigDegree) {
-                    a.big[atomicAdd(a.n_big, 1)] = v;
+                    const int bi = atomicAdd(a.n_big, 1);
+                    a.big[bi] = v;
+                    if (a.hub_cnt) a.big_from[bi] = own;
                 } else {
-                    for (int e = beg; e < end; ++e) a.next_active[a.adjncy[e]] = 1;
+                    for (int e = beg; e < end; ++e) {
+                        const int u = a.adjncy[e];
+                        a.next_active[u] = 1;
+                        if (a.hub_cnt) {
+                            const int slot = a.hub_slot[u];
+                            if (slot >= 0) {
+                                atomicSub(&a.hub_cnt[slot + own], a.adjwgt[e]);
+                                atomicAdd(&a.hub_cnt[slot + t], a.adjwgt[e]);
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -879,9 +900,29 @@ __global__ __launch_bounds__(256) void mark_big_neighbours(LpArgs a) {
     const int nb = *a.n_big;
     for (int i = blockIdx.x; i < nb; i += gridDim.x) {
         const int v = a.big[i];
-        for (int e = a.xadj[v] + threadIdx.x; e < a.xadj[v + 1]; e += 256)
-            a.next_active[a.adjncy[e]] = 1;
+        const int to = a.label[v];
+        const int from = a.hub_cnt ? a.big_from[i] : 0;
+        for (int e = a.xadj[v] + threadIdx.x; e < a.xadj[v + 1]; e += 256) {
+            const int u = a.adjncy[e];
+            a.next_active[u] = 1;
+            if (a.hub_cnt) {
+                const int slot = a.hub_slot[u];
+                if (slot >= 0) {
+                    atomicSub(&a.hub_cnt[slot + from], a.adjwgt[e]);
+                    atomicAdd(&a.hub_cnt[slot + to], a.adjwgt[e]);
+                }
+            }
+        }
     }
+}
+
+__global__ void set_hub_slots(const int* __restrict__ list, const int* __restrict__ multi,
+                              int n_multi, const int* __restrict__ dense_off,
+                              int* __restrict__ hub_slot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_multi) return;
+    const int i = multi[t];
+    hub_slot[list[i]] = dense_off[i];
 }
 
 // ---------------------------------------------------------------- binning
@@ -1078,6 +1119,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         hubs += hub_n_[sr];
     }
     hub_total_ = 0;
+    dense_total_ = 0;
     for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = 0;
     if (hubs == 0) {
         KH_CHECK(hipGetLastError());
@@ -1095,10 +1137,12 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     KH_CHECK(hipMemcpyAsync(deg.data(), hub_deg_.p, sizeof(int) * hubs, hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
     if (dense_hub_) {
-        // chunked dense refinement: chunk prefix per sub-round and global counters for every
-        // hub of more than one chunk
+        // chunked dense refinement: chunk prefix per sub-round and global counters for the hubs
+        // that keep them (every hub when hubs x labels counters fit in kAllHubCounters, else the
+        // hubs of more than one chunk)
         std::vector<int> prefix, multi, doff(hubs, -1);
         long long dtotal = 0;
+        dense_all_ = (long long)hubs * num_labels <= kAllHubCounters;
         for (int sr = 0; sr < S; ++sr) {
             chunk_base_[sr] = (int)prefix.size();
             multi_base_[sr] = (int)multi.size();
@@ -1107,7 +1151,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                 const int nch = (deg[hub_base_[sr] + i] + kHubChunk - 1) / kHubChunk;
                 prefix.push_back(c);
                 c += nch;
-                if (nch > 1) {
+                if (nch > 1 || dense_all_) {
                     doff[hub_base_[sr] + i] = (int)dtotal;
                     dtotal += num_labels;
                     multi.push_back(i);
@@ -1118,11 +1162,15 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
             hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
         }
         KH_REQUIRE(dtotal < (1LL << 31), "too many refinement hub counters");
+        dense_total_ = dtotal;
         chunk_prefix_.reserve((long long)prefix.size());
         multi_.reserve(std::max<long long>((long long)multi.size(), 1));
         dense_off_.reserve(hubs);
         dense_cnt_.reserve(std::max<long long>(dtotal, 1));
         fill_async(dense_cnt_.p, std::max<long long>(dtotal, 1), 0, s);
+        hub_slot_.reserve(g.n);
+        big_from_.reserve(g.n);
+        fill_async(hub_slot_.p, g.n, -1, s);
         KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
                                 hipMemcpyHostToDevice, s));
         if (!multi.empty())
@@ -1130,6 +1178,13 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                                     hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(dense_off_.p, doff.data(), sizeof(int) * hubs,
                                 hipMemcpyHostToDevice, s));
+        for (int sr = 0; sr < S; ++sr)
+            if (hub_multi_n_[sr] > 0)
+                hipLaunchKernelGGL(set_hub_slots, dim3(grid_for(hub_multi_n_[sr], 256)),
+                                   dim3(256), 0, s,
+                                   lists_.p + host_offsets[sr * kNumBins + kNumBins - 1],
+                                   multi_.p + multi_base_[sr], hub_multi_n_[sr],
+                                   dense_off_.p + hub_base_[sr], hub_slot_.p);
         KH_CHECK(hipStreamSynchronize(s));  // the host vectors must outlive the copies
         KH_CHECK(hipGetLastError());
         return;
@@ -1270,9 +1325,27 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.prop_ok = prop_ok_.p;
     a.trav = trav_.p;
     a.big = big_.p;
+    // Refinement hubs of more than one chunk keep per-block connection counters across the
+    // rounds: one scan builds them, every applied move updates its hub neighbours' counters, and
+    // the hubs then decide from the counters instead of rescanning their edges each round.
+    const bool incremental = !cl && dense_hub_ && dense_total_ > 0;
+    a.hub_slot = incremental ? hub_slot_.p : nullptr;
+    a.hub_cnt = incremental ? dense_cnt_.p : nullptr;
+    a.big_from = incremental ? big_from_.p : nullptr;
     uint8_t* act = active_.p;
     uint8_t* nxt = next_active_.p;
     const std::string base = std::string(phase ? phase : (cl ? "lp_cluster" : "lp_refine"));
+    if (incremental && rounds > 0) {
+        ProfScope ps(prof, base + "_hub_counters", 0.0, 0, s);
+        fill_async(dense_cnt_.p, dense_total_, 0, s);
+        const int copies = std::min(kHubDenseThreads / 64, kHubDenseSlots / num_labels_);
+        const size_t lds = sizeof(int) * (size_t)copies * num_labels_;
+        for (int sr = 0; sr < S; ++sr) {
+            if (hub_multi_n_[sr] == 0) continue;
+            const int l = sr * kNumBins + kNumBins - 1;
+            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lists_.p + host_offsets[l], host_counts[l], chunk_prefix_.p + chunk_base_[sr], num_labels_, dense_off_.p + hub_base_[sr], dense_cnt_.p, 1, 0);
+        }
+    }
     const int nl = num_labels_;
     for (int r = 0; r < rounds; ++r) {
         a.tie_seed = tie_seed_of(seed, r);
@@ -1320,11 +1393,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             const int* dof = dense_off_.p + hub_base_[sr];
                             const int copies = std::min(kHubDenseThreads / 64, kHubDenseSlots / nl);
                             const size_t lds = sizeof(int) * (size_t)copies * nl;
-                            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p);
+                            if (!(incremental && dense_all_))  // otherwise every hub decides from counters
+                                hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p, 0, incremental ? 1 : 0);
                             ps.close();
                             if (hub_multi_n_[sr] > 0) {
                                 ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_multi_n_[sr], s);
-                                hipLaunchKernelGGL(hub_dense_decide, dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, multi_.p + multi_base_[sr], hub_multi_n_[sr], nl, dof, dense_cnt_.p);
+                                hipLaunchKernelGGL(hub_dense_decide, dim3(hub_multi_n_[sr]), dim3(256), 0, s, a, lst, multi_.p + multi_base_[sr], hub_multi_n_[sr], nl, dof, dense_cnt_.p, incremental ? 1 : 0);
                             }
                             break;
                         }

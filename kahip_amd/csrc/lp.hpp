@@ -33,6 +33,12 @@ struct LpArgs {
     int* big;                  // refinement: moved nodes of degree > kBigDegree
     int* n_big;
     unsigned long long* trav;  // refinement: traversed directed edges, 64 spread counters
+    // Refinement with incremental hub counters (nullptr otherwise): hub_slot[v] is the offset of
+    // node v's per-block connection counters in hub_cnt, or -1; moves update the counters of
+    // their hub neighbours; big_from[i] is the old block of big mover big[i].
+    const int* hub_slot;
+    int* hub_cnt;
+    int* big_from;
 };
 
 struct LpStats {
@@ -98,6 +104,9 @@ private:
     DevBuf<uint8_t> active_, next_active_;
     DevBuf<int> hub_key_, hub_val_;
     DevBuf<int> dense_off_, dense_cnt_;  // chunked dense refinement hubs: per-hub counters
+    DevBuf<int> hub_slot_, big_from_;
+    long long dense_total_ = 0;
+    bool dense_all_ = false;
     long long hub_total_ = 0;
 };
 

@@ -40,6 +40,7 @@ def graphs(rgg15, delaunay15, weighted132):
         "star700w": star(700, weighted=True),
         "rmat12": K.gen_rmat(12, 16 << 12, 7),
         "star10k": star(10000),
+        "rmat15": K.gen_rmat(15, 16 << 15, 9),
     }
     return out
 
@@ -148,7 +149,7 @@ def test_contract_chunked_tier(case):
 
 REFINE_CASES = [("er200w", 2, 0), ("rgg15", 2, 1), ("rgg15", 64, 2), ("delaunay15", 8, 3),
                 ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6), ("star10k", 4, 7),
-                ("star10k", 5000, 8), ("rmat12", 3000, 9)]
+                ("star10k", 5000, 8), ("rmat12", 3000, 9), ("rmat15", 64, 10), ("rmat15", 8, 11)]
 
 
 @pytest.mark.parametrize("name,k,seed", REFINE_CASES)
