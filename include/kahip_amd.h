@@ -37,6 +37,7 @@ typedef struct {
                             the caller's numbering */
     int ensemble;        /* LP clusterings per level, overlaid (1 = single clustering; the
                             STRONGSOCIAL preset uses 18/17/15/7/3 for k <= 3/7/15/31/more) */
+    int ip_tries;        /* BFS-grown starts per bisection of the initial partitioner (32) */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);

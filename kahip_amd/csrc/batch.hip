@@ -429,6 +429,7 @@ void partition_batch(Engine& eng, const DGraph& g0, int G, const int* node_off,
                 ip.seed = ((uint64_t)(uint32_t)p.seed * 0x9E3779B97F4A7C15ULL + 0x1234567ULL) ^
                           ((uint64_t)hash3((uint32_t)p.seed, 0xB7u, (uint32_t)i) << 1);
                 ip.threads = 1;
+                ip.bisection_tries = p.ip_tries;
                 initial_partition(sub, ip, ip_part[i]);
             }
         };

@@ -70,6 +70,8 @@ PartitionParams to_params(const kahip_params* q) {
     p.mlp = reinterpret_cast<const MlpWeights*>(q->mlp_weights);
     p.reorder = q->reorder;
     p.ensemble = q->ensemble;
+    p.ip_tries = q->ip_tries;
+    KH_REQUIRE(p.ip_tries >= 1, "ip_tries must be >= 1");
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
@@ -241,6 +243,7 @@ void kahip_default_params(kahip_params* p) {
     p->mlp_weights = nullptr;
     p->reorder = d.reorder;
     p->ensemble = d.ensemble;
+    p->ip_tries = d.ip_tries;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }

@@ -279,6 +279,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     ip.repetitions = p.ip_repetitions > 0 ? p.ip_repetitions : ip_repetitions_for(p.k);
     ip.seed = (uint64_t)(uint32_t)p.seed * 0x9E3779B97F4A7C15ULL + 0x1234567ULL;
     ip.threads = p.ip_threads;
+    ip.bisection_tries = p.ip_tries;
     initial_partition(hg, ip, hpart);
     DevBuf<int> part_c, part_f;
     part_c.reserve(std::max(cg.n, 1));

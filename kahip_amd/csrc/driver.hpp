@@ -29,6 +29,7 @@ struct PartitionParams {
     int stop_factor = 5000;    // app/configuration.h:497 (num_vert_stop_factor)
     int ip_repetitions = -1;   // -1: the reference's count (initial_partitioning.cpp:69-81)
     int ip_threads = 0;
+    int ip_tries = 32;         // BFS-grown starts per bisection of the initial partitioner
     BinConfig bins;
     bool profile = false;
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels

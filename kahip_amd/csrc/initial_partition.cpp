@@ -20,7 +20,6 @@
 #include <numeric>
 #include <queue>
 #include <thread>
-#include <tuple>
 
 namespace kahip {
 
@@ -173,51 +172,130 @@ inline long long overload(const long long w[2], const long long maxw[2]) {
     return std::max(0LL, w[0] - maxw[0]) + std::max(0LL, w[1] - maxw[1]);
 }
 
+// Addressable binary max-heap of the FM queue of one side: one entry per node, ordered by
+// (gain, random tiebreak, node). A gain change draws a new tiebreak and moves the entry in place.
+struct FmHeap {
+    std::vector<int> h;  // node ids in heap order
+    int* pos = nullptr;  // pos[v]: index of v in h, -1 if absent (shared by both sides)
+    const long long* gain = nullptr;
+    const uint64_t* key = nullptr;  // tie << 32 | node
+    bool less(int a, int b) const {
+        return gain[a] < gain[b] || (gain[a] == gain[b] && key[a] < key[b]);
+    }
+    void up(int i) {
+        const int v = h[i];
+        while (i > 0) {
+            const int p = (i - 1) >> 1;
+            if (!less(h[p], v)) break;
+            h[i] = h[p];
+            pos[h[i]] = i;
+            i = p;
+        }
+        h[i] = v;
+        pos[v] = i;
+    }
+    void down(int i) {
+        const int v = h[i];
+        const int sz = (int)h.size();
+        while (true) {
+            int c = 2 * i + 1;
+            if (c >= sz) break;
+            if (c + 1 < sz && less(h[c], h[c + 1])) ++c;
+            if (!less(v, h[c])) break;
+            h[i] = h[c];
+            pos[h[i]] = i;
+            i = c;
+        }
+        h[i] = v;
+        pos[v] = i;
+    }
+    void insert(int v) {
+        h.push_back(v);
+        up((int)h.size() - 1);
+    }
+    void update(int v) {  // after key/gain of v changed
+        const int i = pos[v];
+        up(i);
+        down(pos[v]);
+    }
+    void pop() {
+        const int v = h[0];
+        pos[v] = -1;
+        const int last = h.back();
+        h.pop_back();
+        if (!h.empty()) {
+            h[0] = last;
+            pos[last] = 0;
+            down(0);
+        }
+    }
+};
+
+// Buffers of fm_pass, kept per thread across calls (the tries call it thousands of times on
+// graphs of a few hundred nodes, where allocation would dominate).
+struct FmScratch {
+    std::vector<long long> gain;
+    std::vector<uint64_t> key;
+    std::vector<int> pos, moved;
+    std::vector<char> locked;
+    std::vector<int> heap[2];
+};
+
 // One FM pass with rollback to the best prefix. State order: lower overload first, then lower
-// cut. Returns true when the state improved.
+// cut. Returns true when the state improved. Queues: boundary nodes of each side by gain; a node
+// enters its queue when it first has a moved neighbour or at the start if it is on the boundary.
 bool fm_pass(const HostGraph& g, Bisection& b, const long long maxw[2], int stall_limit,
-             Rng& rng) {
+             int stall_pct, Rng& rng) {
+    static thread_local FmScratch sc;
     const int n = g.n;
-    std::vector<long long> gain(n, 0);
-    std::vector<char> locked(n, 0), boundary(n, 0);
+    // moved into locals for the pass (and back at the end), so that the compiler can keep their
+    // data pointers in registers across the char stores into b.side
+    std::vector<long long> gain = std::move(sc.gain);
+    std::vector<uint64_t> key = std::move(sc.key);
+    std::vector<int> pos = std::move(sc.pos);
+    std::vector<int> moved = std::move(sc.moved);
+    std::vector<char> locked = std::move(sc.locked);
+    FmHeap pq[2];
+    pq[0].h = std::move(sc.heap[0]);
+    pq[1].h = std::move(sc.heap[1]);
+    gain.assign(n, 0);
+    key.resize(n);
+    pos.assign(n, -1);
+    locked.assign(n, 0);
+    moved.clear();
+    for (int s = 0; s < 2; ++s) {
+        pq[s].h.clear();
+        pq[s].pos = pos.data();
+        pq[s].gain = gain.data();
+        pq[s].key = key.data();
+    }
+    int nb = 0;
     for (int v = 0; v < n; ++v) {
         long long gv = 0;
+        bool bnd = false;
         for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
             if (b.side[g.adjncy[e]] != b.side[v]) {
                 gv += g.adjwgt[e];
-                boundary[v] = 1;
+                bnd = true;
             } else {
                 gv -= g.adjwgt[e];
             }
         }
         gain[v] = gv;
+        if (bnd) {
+            key[v] = ((uint64_t)(uint32_t)rng.next() << 32) | (uint32_t)v;
+            pq[(int)b.side[v]].insert(v);
+            ++nb;
+        }
     }
-    // max-heaps of (gain, random tiebreak, node) per side; stale entries skipped lazily
-    using Entry = std::tuple<long long, uint32_t, int>;
-    std::priority_queue<Entry> pq[2];
-    for (int v = 0; v < n; ++v)
-        if (boundary[v]) pq[(int)b.side[v]].emplace(gain[v], (uint32_t)rng.next(), v);
-    std::vector<int> moved;
+    if (stall_pct > 0) stall_limit = std::max(15, (int)((long long)stall_pct * nb / 100));
     long long cut = b.cut, w[2] = {b.w[0], b.w[1]};
     long long best_cut = cut, best_over = overload(w, maxw);
     size_t best_len = 0;
     int since_best = 0;
-    auto top_of = [&](int s, long long& gv) -> int {
-        while (!pq[s].empty()) {
-            auto [gg, r, v] = pq[s].top();
-            (void)r;
-            if (locked[v] || b.side[v] != s || gg != gain[v]) {
-                pq[s].pop();
-                continue;
-            }
-            gv = gg;
-            return v;
-        }
-        return -1;
-    };
     while (true) {
-        long long g0 = 0, g1 = 0;
-        const int v0 = top_of(0, g0), v1 = top_of(1, g1);
+        const int v0 = pq[0].h.empty() ? -1 : pq[0].h[0];
+        const int v1 = pq[1].h.empty() ? -1 : pq[1].h[0];
         // a move from s to 1-s is allowed if the target stays within its bound or the source
         // is overloaded
         auto ok = [&](int v, int s) {
@@ -228,7 +306,7 @@ bool fm_pass(const HostGraph& g, Bisection& b, const long long maxw[2], int stal
         const bool a0 = ok(v0, 0), a1 = ok(v1, 1);
         int v = -1, s = 0;
         if (a0 && a1) {
-            if (g0 > g1 || (g0 == g1 && w[0] >= w[1])) {
+            if (gain[v0] > gain[v1] || (gain[v0] == gain[v1] && w[0] >= w[1])) {
                 v = v0;
                 s = 0;
             } else {
@@ -260,7 +338,11 @@ bool fm_pass(const HostGraph& g, Bisection& b, const long long maxw[2], int stal
                 gain[u] += d;
             else
                 gain[u] -= d;
-            pq[(int)b.side[u]].emplace(gain[u], (uint32_t)rng.next(), u);
+            key[u] = ((uint64_t)(uint32_t)rng.next() << 32) | (uint32_t)u;
+            if (pos[u] < 0)
+                pq[(int)b.side[u]].insert(u);
+            else
+                pq[(int)b.side[u]].update(u);
         }
         const long long ov = overload(w, maxw);
         if (ov < best_over || (ov == best_over && cut < best_cut)) {
@@ -285,12 +367,20 @@ bool fm_pass(const HostGraph& g, Bisection& b, const long long maxw[2], int stal
     b.w[0] = w[0];
     b.w[1] = w[1];
     b.cut = best_cut;
+    sc.gain = std::move(gain);
+    sc.key = std::move(key);
+    sc.pos = std::move(pos);
+    sc.moved = std::move(moved);
+    sc.locked = std::move(locked);
+    sc.heap[0] = std::move(pq[0].h);
+    sc.heap[1] = std::move(pq[1].h);
     return improved;
 }
 
-void fm_refine(const HostGraph& g, Bisection& b, const long long maxw[2], int stall, Rng& rng) {
+void fm_refine(const HostGraph& g, Bisection& b, const long long maxw[2], int stall, int pct,
+               Rng& rng) {
     for (int pass = 0; pass < 8; ++pass)
-        if (!fm_pass(g, b, maxw, stall, rng)) break;
+        if (!fm_pass(g, b, maxw, stall, pct, rng)) break;
 }
 
 // BFS growing of side 0 from random starts until it reaches target0.
@@ -356,7 +446,7 @@ std::vector<char> ml_bisect(const HostGraph& g, long long target0, const long lo
     bool have = false;
     for (int t = 0; t < p.bisection_tries; ++t) {
         Bisection b = grow(*cur, target0, rng);
-        fm_refine(*cur, b, maxw, p.fm_stall, rng);
+        fm_refine(*cur, b, maxw, p.fm_stall, p.fm_pct_coarsest, rng);
         if (!have || better_bis(b, best, maxw)) {
             best = std::move(b);
             have = true;
@@ -371,7 +461,7 @@ std::vector<char> ml_bisect(const HostGraph& g, long long target0, const long lo
         b.w[0] = best.w[0];
         b.w[1] = best.w[1];
         b.cut = best.cut;
-        fm_refine(fine, b, maxw, p.fm_stall, rng);
+        fm_refine(fine, b, maxw, p.fm_stall, p.fm_pct_levels, rng);
         best = std::move(b);
     }
     return best.side;
@@ -545,7 +635,9 @@ void initial_partition(const HostGraph& g, const IpParams& p, std::vector<int>& 
             for (int e = core.xadj[v]; e < core.xadj[v + 1]; ++e)
                 if (results[r][v] != results[r][core.adjncy[e]]) cuts[r] += core.adjwgt[e];
     };
-    {
+    if (threads <= 1) {  // the caller parallelises (e.g. over the graphs of a batch)
+        for (int r = 0; r < reps; ++r) run_rep(r);
+    } else {
         std::vector<std::thread> pool;
         for (int r = 0; r < reps; ++r) pool.emplace_back(run_rep, r);
         for (auto& t : pool) t.join();

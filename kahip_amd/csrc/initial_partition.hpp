@@ -20,7 +20,9 @@ struct IpParams {
     int bisection_tries = 32;   // BFS-grown starts per bisection, on its coarsest (<= stop_nodes) graph
     int coarsen_factor = 12;    // cluster bound of the bisection coarsening: ceil(max side / 12)
     int stop_nodes = 400;       // bisection coarsening stops below this many nodes
-    int fm_stall = 500;         // FM stops after this many moves without improvement
+    int fm_stall = 500;         // FM stops after this many moves without improvement, unless:
+    int fm_pct_coarsest = 0;    // > 0: limit = max(15, pct % of the boundary nodes), on the
+    int fm_pct_levels = 0;      //   coarsest graph / on the uncoarsening levels (two_way_fm.cpp:69-71)
     uint64_t seed = 0;
     int threads = 0;            // 0: hardware concurrency (capped at 16)
 };
