@@ -361,28 +361,16 @@ __device__ inline void block_reduce(int& conn_own, Cand& c) {
 // the incremental scheme); incremental: those hubs are skipped (the moves keep their counters up
 // to date).
 __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
-    LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    int num_labels, const int* __restrict__ dense_off, int* __restrict__ dcnt, int build,
-    int incremental) {
+    LpArgs a, const int4* __restrict__ cmap, int num_labels, const int* __restrict__ dense_off,
+    int* __restrict__ dcnt, int build, int incremental) {
     extern __shared__ int cnt[];  // copies x num_labels counters (dynamic: small k, more blocks)
-    __shared__ int hub;
-    const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
-        }
-        hub = lo;
-    }
-    __syncthreads();
-    const int gi = hub;
-    const int v = list[gi];
-    const int nchunks = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int4 cm = cmap[blockIdx.x];  // {hub index, chunk of the hub, chunks of the hub, node}
+    const int gi = cm.x;
+    const int v = cm.w;
     const bool counted = dense_off[gi] >= 0;  // the hub keeps global counters
     if (build ? !counted : (a.active[v] == 0 || (incremental && counted)))
         return;  // uniform per block
-    const int c_idx = chunk - chunk_prefix[gi];
+    const int c_idx = cm.y;
     const int beg = a.xadj[v];
     const int end = a.xadj[v + 1];
     const int e0 = beg + c_idx * kHubChunk;
@@ -480,32 +468,27 @@ __global__ __launch_bounds__(256) void hub_dense_decide(
 // sum) entries are written out grouped by bucket(label), one bucket per later workgroup.
 template <LpMode M>
 __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
-    LpArgs a, const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    int2* __restrict__ partials, const long long* __restrict__ offm_base, int* __restrict__ offm,
+    LpArgs a, const int4* __restrict__ cmap, int2* __restrict__ partials,
+    const long long* __restrict__ offm_base, int* __restrict__ offm,
     const int* __restrict__ ct_base, int* __restrict__ cntT) {
     __shared__ int K[kHubCap];
     __shared__ int V[kHubCap];
-    __shared__ int hub;
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
-        }
-        hub = lo;
-    }
-    __syncthreads();
-    const int gi = hub;
-    const int v = list[gi];
+    const int4 cm = cmap[chunk];  // {hub index, chunk of the hub, chunks of the hub, node}
+    const int gi = cm.x;
+    const int v = cm.w;
     if (M == LpMode::Refine && a.active[v] == 0) return;  // uniform per block
-    const int nchunks = chunk_prefix[gi + 1] - chunk_prefix[gi];
-    const int c_idx = chunk - chunk_prefix[gi];
+    const int nchunks = cm.z;
+    const int c_idx = cm.y;
     const int beg = a.xadj[v];
     const int end = a.xadj[v + 1];
     const int e0 = beg + c_idx * kHubChunk;
     const int e1 = min(end, e0 + kHubChunk);
-    for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
+    // a hub of one chunk uses a table of the next power of two >= 2 x its degree
+    int cap = kHubCap;
+    if (nchunks == 1)
+        while (cap > 256 && cap >= 4 * (e1 - e0)) cap >>= 1;
+    for (int i = threadIdx.x; i < cap; i += kHubThreads) {
         K[i] = -1;
         V[i] = 0;
     }
@@ -527,14 +510,14 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
         const int lab = labs[j];
         if (lab < 0) continue;
         const int w = ws[j];
-        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(kHubCap - 1);
+        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(cap - 1);
         while (true) {
             const int old = atomicCAS(&K[h], -1, lab);
             if (old == -1 || old == lab) {
                 atomicAdd(&V[h], w);
                 break;
             }
-            h = (h + 1) & (uint32_t)(kHubCap - 1);
+            h = (h + 1) & (uint32_t)(cap - 1);
         }
     }
     __syncthreads();
@@ -596,7 +579,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     const int nw = a.vwgt[v];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (int i = threadIdx.x; i < kHubCap; i += kHubThreads) {
+    for (int i = threadIdx.x; i < cap; i += kHubThreads) {
         const int lab = K[i];
         if (lab < 0) continue;
         const int conn = V[i];
@@ -620,25 +603,15 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
 // position (posT = exclusive scan of the bucket-major counts), so that every bucket of a hub is
 // one contiguous range.
 __global__ __launch_bounds__(256) void hub_scatter(
-    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const long long* __restrict__ offm_base, const int* __restrict__ offm,
-    const int* __restrict__ ct_base, const int* __restrict__ posT,
+    const int4* __restrict__ cmap, const long long* __restrict__ offm_base,
+    const int* __restrict__ offm, const int* __restrict__ ct_base, const int* __restrict__ posT,
     const int2* __restrict__ partials, int2* __restrict__ bucketed) {
-    __shared__ int hub;
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = count;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
-        }
-        hub = lo;
-    }
-    __syncthreads();
-    const int gi = hub;
-    const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int4 cm = cmap[chunk];
+    const int gi = cm.x;
+    const int nch = cm.z;
     if (nch <= 1) return;
-    const int c_idx = chunk - chunk_prefix[gi];
+    const int c_idx = cm.y;
     const int P = min(kBucketsPerChunk * nch, kMaxBuckets);
     const int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
     const int* pos = posT + ct_base[gi] + c_idx;
@@ -660,7 +633,7 @@ __global__ __launch_bounds__(256) void hub_scatter(
 template <LpMode M, bool SMALL>
 __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
     LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
-    int n_tasks, const int* __restrict__ task_prefix, const int* __restrict__ chunk_prefix,
+    int n_tasks, const int4* __restrict__ tmap, const int* __restrict__ chunk_prefix,
     const int* __restrict__ ct_base, const int* __restrict__ posT,
     const int2* __restrict__ bucketed, const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
@@ -670,29 +643,20 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
     constexpr int CAP = SMALL ? kSmallCap : kFinCap;
     __shared__ int K[CAP];
     __shared__ int V[CAP];
-    __shared__ int sh_mi;
     const int n_iter = SMALL ? n_tasks : *n_big_tasks;
     for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
         const int task = SMALL ? it : big_tasks[it];
-        if (threadIdx.x == 0) {
-            int lo = 0, hi = n_multi;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (task_prefix[mid] <= task) lo = mid; else hi = mid;
-            }
-            sh_mi = lo;
-        }
-        __syncthreads();
-        const int mi = sh_mi;
-        const int gi = multi[mi];
+        const int4 tm = tmap[task];  // {multi-chunk hub index, bucket, buckets of the hub, hub index}
+        const int mi = tm.x;
+        const int gi = tm.w;
         const int v = list[gi];
         if (M == LpMode::Refine && a.active[v] == 0) {  // uniform per block
             __syncthreads();
             continue;
         }
         const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
-        const int P = task_prefix[mi + 1] - task_prefix[mi];
-        const int b = task - task_prefix[mi];
+        const int P = tm.z;
+        const int b = tm.y;
         const int r0 = posT[ct_base[gi] + b * nch];
         const int r1 = posT[ct_base[gi] + (b + 1) * nch];
         const int total = r1 - r0;
@@ -985,6 +949,27 @@ __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, 
     if (l >= 0) lists[offsets[l] + base[l] + r] = v;
 }
 
+// Block descriptors of the hub kernels, built once per level: chunk c of the sub-round's hub
+// list -> {hub index, chunk of the hub, chunks of the hub, node}.
+__global__ void build_chunk_map(const int* __restrict__ list, int count,
+                                const int* __restrict__ chunk_prefix, int4* __restrict__ cmap) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= count) return;
+    const int c0 = chunk_prefix[gi], c1 = chunk_prefix[gi + 1];
+    const int v = list[gi];
+    for (int c = c0; c < c1; ++c) cmap[c] = make_int4(gi, c - c0, c1 - c0, v);
+}
+
+// (hub, bucket) task t of a sub-round -> {multi-chunk hub index, bucket, buckets, hub index}.
+__global__ void build_task_map(const int* __restrict__ multi, int n_multi,
+                               const int* __restrict__ task_prefix, int4* __restrict__ tmap) {
+    const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mi >= n_multi) return;
+    const int t0 = task_prefix[mi], t1 = task_prefix[mi + 1];
+    const int gi = multi[mi];
+    for (int t = t0; t < t1; ++t) tmap[t] = make_int4(mi, t - t0, t1 - t0, gi);
+}
+
 __global__ void hub_degrees(const int* __restrict__ list, int count, const int* __restrict__ xadj,
                             int* __restrict__ deg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1178,6 +1163,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                                     hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(dense_off_.p, doff.data(), sizeof(int) * hubs,
                                 hipMemcpyHostToDevice, s));
+        build_chunk_maps(s);
         for (int sr = 0; sr < S; ++sr)
             if (hub_multi_n_[sr] > 0)
                 hipLaunchKernelGGL(set_hub_slots, dim3(grid_for(hub_multi_n_[sr], 256)),
@@ -1275,6 +1261,22 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     if (!multi.empty())
         KH_CHECK(hipMemcpyAsync(multi_.p, multi.data(), sizeof(int) * multi.size(),
                                 hipMemcpyHostToDevice, s));
+    build_chunk_maps(s);
+    {
+        long long all_tasks = 0;  // one descriptor per (hub, bucket) task of every sub-round
+        for (int sr = 0; sr < S; ++sr) all_tasks += hub_tasks_[sr];
+        tmap_.reserve(std::max<long long>(all_tasks, 1));
+        long long tb = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            KH_REQUIRE(tb + hub_tasks_[sr] <= all_tasks, "task map overflow");
+            if (hub_multi_n_[sr] > 0)
+                hipLaunchKernelGGL(build_task_map, dim3(grid_for(hub_multi_n_[sr], 256)), dim3(256),
+                                   0, s, multi_.p + multi_base_[sr], hub_multi_n_[sr],
+                                   task_prefix_.p + task_base_[sr], tmap_.p + tb);
+            tmap_base_[sr] = (int)tb;
+            tb += hub_tasks_[sr];
+        }
+    }
     if (hub_total_ > 0) {
         hub_key_.reserve(hub_total_);
         hub_val_.reserve(hub_total_);
@@ -1287,6 +1289,22 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     bucketed_.reserve((long long)max_chunks * kHubChunk);
     KH_CHECK(hipStreamSynchronize(s));  // the host vectors above must outlive the copies
     KH_CHECK(hipGetLastError());
+}
+
+void LpWorkspace::build_chunk_maps(hipStream_t s) {
+    long long total = 0;
+    for (int sr = 0; sr < subrounds_; ++sr) total += hub_chunks_[sr];
+    cmap_.reserve(std::max<long long>(total, 1));
+    long long cb = 0;
+    for (int sr = 0; sr < subrounds_; ++sr) {
+        KH_REQUIRE(cb + hub_chunks_[sr] <= total, "chunk map overflow");
+        cmap_base_[sr] = (int)cb;
+        if (hub_n_[sr] > 0)
+            hipLaunchKernelGGL(build_chunk_map, dim3(grid_for(hub_n_[sr], 256)), dim3(256), 0, s,
+                               lists_.p + host_offsets[sr * kNumBins + kNumBins - 1], hub_n_[sr],
+                               chunk_prefix_.p + chunk_base_[sr], cmap_.p + cb);
+        cb += hub_chunks_[sr];
+    }
 }
 
 static const char* kBinName[kNumBins] = {"g8", "g16", "g32", "g64", "wave", "hub"};
@@ -1343,7 +1361,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         for (int sr = 0; sr < S; ++sr) {
             if (hub_multi_n_[sr] == 0) continue;
             const int l = sr * kNumBins + kNumBins - 1;
-            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lists_.p + host_offsets[l], host_counts[l], chunk_prefix_.p + chunk_base_[sr], num_labels_, dense_off_.p + hub_base_[sr], dense_cnt_.p, 1, 0);
+            hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, cmap_.p + cmap_base_[sr], num_labels_, dense_off_.p + hub_base_[sr], dense_cnt_.p, 1, 0);
         }
     }
     const int nl = num_labels_;
@@ -1389,12 +1407,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         break;
                     default: {
                         if (!cl && dense_hub_) {
-                            const int* cp = chunk_prefix_.p + chunk_base_[sr];
+                            const int4* cm = cmap_.p + cmap_base_[sr];
                             const int* dof = dense_off_.p + hub_base_[sr];
                             const int copies = std::min(kHubDenseThreads / 64, kHubDenseSlots / nl);
                             const size_t lds = sizeof(int) * (size_t)copies * nl;
                             if (!(incremental && dense_all_))  // otherwise every hub decides from counters
-                                hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, lst, cnt, cp, nl, dof, dense_cnt_.p, 0, incremental ? 1 : 0);
+                                hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, cm, nl, dof, dense_cnt_.p, 0, incremental ? 1 : 0);
                             ps.close();
                             if (hub_multi_n_[sr] > 0) {
                                 ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_multi_n_[sr], s);
@@ -1403,16 +1421,18 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             break;
                         }
                         const int* cp = chunk_prefix_.p + chunk_base_[sr];
+                        const int4* cm = cmap_.p + cmap_base_[sr];
                         const long long* to = tab_off_.p + hub_base_[sr];
                         const long long* ob = offm_base_.p + hub_base_[sr];
                         const int* ctb = ct_base_.p + hub_base_[sr];
-                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
-                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, lst, cnt, cp, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         ps.close();  // the hub scope times propose_hub_chunk alone
                         if (hub_multi_n_[sr] > 0) {
                             ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);
                             const int* ml = multi_.p + multi_base_[sr];
                             const int* tp = task_prefix_.p + task_base_[sr];
+                            const int4* tm = tmap_.p + tmap_base_[sr];
                             const int nm = hub_multi_n_[sr];
                             Cand* br = reinterpret_cast<Cand*>(bres_.p);
                             const int nt = hub_tasks_[sr];
@@ -1420,16 +1440,16 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             size_t sb = scan_tmp_bytes_;
                             KH_CHECK(rocprim::exclusive_scan(scan_tmp_.p, sb, cnt_t_.p + ct_sr_base_[sr], pos_t_.p + ct_sr_base_[sr], 0, (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
                             const int* pt = pos_t_.p;
-                            hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, lst, cnt, cp, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
+                            hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, cm, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tp, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

@@ -70,6 +70,7 @@ public:
     int host_offsets[kMaxSub * kNumBins] = {};
 
 private:
+    void build_chunk_maps(hipStream_t s);  // cmap_ of every sub-round (after chunk_prefix_)
     int subrounds_ = 1;
     int num_labels_ = 0;
     bool dense_wave_ = false, dense_hub_ = false;
@@ -79,6 +80,8 @@ private:
     int hub_multi_n_[kMaxSub] = {};
     int hub_base_[kMaxSub] = {};    // offset into tab_off_/tab_cap_ (list order)
     int chunk_base_[kMaxSub] = {};  // offset into chunk_prefix_
+    int cmap_base_[kMaxSub] = {};   // offset into cmap_ (chunks of the earlier sub-rounds)
+    int tmap_base_[kMaxSub] = {};   // offset into tmap_ (tasks of the earlier sub-rounds)
     int multi_base_[kMaxSub] = {};  // offset into multi_
     int task_base_[kMaxSub] = {};   // offset into task_prefix_
     int hub_tasks_[kMaxSub] = {};   // (hub, bucket) tasks per sub-round
@@ -93,6 +96,7 @@ private:
     DevBuf<long long> offm_base_;
     DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_;
     DevBuf<int4> bres_;
+    DevBuf<int4> cmap_, tmap_;  // hub-chunk and (hub, bucket) task descriptors
     DevBuf<uint8_t> fb_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
     DevBuf<unsigned long long> edges_;
