@@ -36,6 +36,8 @@ constexpr int kBlock = 256;
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
+constexpr int kMidThreads = 256;       // hubs of kMidChunk edges or fewer (4 edges per thread)
+constexpr int kMidChunk = 1024;
 constexpr int kHubDenseThreads = 512;  // refinement hub chunks (8 edges per thread)
 constexpr int kHubDense = 4096;   // dense LDS counters per hub (refinement, k <= 4096)
 constexpr long long kAllHubCounters = 1LL << 28;  // counters (ints) for incremental hubs
@@ -463,16 +465,21 @@ __global__ __launch_bounds__(256) void hub_dense_decide(
 }
 
 // ---------------------------------------------------------------- hubs: chunked LDS hash
-// One workgroup per chunk of kHubChunk edges of a hub. The chunk's labels are aggregated in an
-// LDS hash table; a hub of one chunk decides right away, otherwise the chunk's (label, partial
-// sum) entries are written out grouped by bucket(label), one bucket per later workgroup.
-template <LpMode M>
-__global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
+// One workgroup per chunk of CHUNK edges of a hub. The chunk's labels are aggregated in an LDS
+// hash table; a hub of one chunk decides right away, otherwise the chunk's (label, partial sum)
+// entries are written out grouped by bucket(label), one bucket per later workgroup. The mid-size
+// instance (kMidThreads, kMidChunk) serves hubs of one chunk only, in small workgroups that
+// share a CU eight at a time.
+template <LpMode M, int THREADS, int CHUNK>
+__global__ __launch_bounds__(THREADS) void propose_hub_chunk(
     LpArgs a, const int4* __restrict__ cmap, int2* __restrict__ partials,
     const long long* __restrict__ offm_base, int* __restrict__ offm,
     const int* __restrict__ ct_base, int* __restrict__ cntT) {
-    __shared__ int K[kHubCap];
-    __shared__ int V[kHubCap];
+    constexpr int CAP = 2 * CHUNK;
+    constexpr bool MULTI = CHUNK == kHubChunk;  // hubs of several chunks come here
+    static_assert(!MULTI || THREADS == kHubThreads, "the multi-chunk path assumes kHubThreads");
+    __shared__ int K[CAP];
+    __shared__ int V[CAP];
     const int chunk = blockIdx.x;
     const int4 cm = cmap[chunk];  // {hub index, chunk of the hub, chunks of the hub, node}
     const int gi = cm.x;
@@ -482,23 +489,23 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     const int c_idx = cm.y;
     const int beg = a.xadj[v];
     const int end = a.xadj[v + 1];
-    const int e0 = beg + c_idx * kHubChunk;
-    const int e1 = min(end, e0 + kHubChunk);
+    const int e0 = beg + c_idx * CHUNK;
+    const int e1 = min(end, e0 + CHUNK);
     // a hub of one chunk uses a table of the next power of two >= 2 x its degree
-    int cap = kHubCap;
+    int cap = CAP;
     if (nchunks == 1)
         while (cap > 256 && cap >= 4 * (e1 - e0)) cap >>= 1;
-    for (int i = threadIdx.x; i < cap; i += kHubThreads) {
+    for (int i = threadIdx.x; i < cap; i += THREADS) {
         K[i] = -1;
         V[i] = 0;
     }
     __syncthreads();
     // load every edge of the thread first (independent label gathers in flight), then insert
-    constexpr int EPT = kHubChunk / kHubThreads;
+    constexpr int EPT = CHUNK / THREADS;
     int labs[EPT], ws[EPT];
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
-        const int e = e0 + threadIdx.x + kHubThreads * j;
+        const int e = e0 + threadIdx.x + THREADS * j;
         labs[j] = e < e1 ? a.adjncy[e] : -1;
         ws[j] = e < e1 ? a.adjwgt[e] : 0;
     }
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
         }
     }
     __syncthreads();
-    if (nchunks > 1) {
+    if constexpr (MULTI) if (nchunks > 1) {
         // The chunk's (label, partial sum) entries, grouped by bucket(label): P buckets per hub,
         // bucket b of chunk c at [offm[c*(P+1)+b], offm[c*(P+1)+b+1]) of the chunk's region.
         __shared__ int bcur[kMaxBuckets + 1];
@@ -579,7 +586,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
     const int nw = a.vwgt[v];
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    for (int i = threadIdx.x; i < cap; i += kHubThreads) {
+    for (int i = threadIdx.x; i < cap; i += THREADS) {
         const int lab = K[i];
         if (lab < 0) continue;
         const int conn = V[i];
@@ -592,7 +599,7 @@ __global__ __launch_bounds__(kHubThreads) void propose_hub_chunk(
         const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
         if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
     }
-    block_reduce<kHubThreads>(conn_own, c);
+    block_reduce<THREADS>(conn_own, c);
     if (threadIdx.x == 0) {
         decide<M>(a, v, own, nw, conn_own, c);
         if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(end - beg));
@@ -960,6 +967,15 @@ __global__ void build_chunk_map(const int* __restrict__ list, int count,
     for (int c = c0; c < c1; ++c) cmap[c] = make_int4(gi, c - c0, c1 - c0, v);
 }
 
+// Hub gi of the mid-size kernel -> mmap[midpos[gi]] = {hub index, 0, 1, node}.
+__global__ void build_mid_map(const int* __restrict__ list, int count,
+                              const int* __restrict__ midpos, int4* __restrict__ mmap) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= count) return;
+    const int m = midpos[gi];
+    if (m >= 0) mmap[m] = make_int4(gi, 0, 1, list[gi]);
+}
+
 // (hub, bucket) task t of a sub-round -> {multi-chunk hub index, bucket, buckets, hub index}.
 __global__ void build_task_map(const int* __restrict__ multi, int n_multi,
                                const int* __restrict__ task_prefix, int4* __restrict__ tmap) {
@@ -1105,7 +1121,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     }
     hub_total_ = 0;
     dense_total_ = 0;
-    for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = 0;
+    for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = mid_n_[sr] = 0;
     if (hubs == 0) {
         KH_CHECK(hipGetLastError());
         return;
@@ -1175,9 +1191,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         KH_CHECK(hipGetLastError());
         return;
     }
-    std::vector<int> prefix, multi, tasks;
+    std::vector<int> prefix, multi, tasks, midpos(hubs, -1);
     std::vector<long long> toff(hubs, 0), tcap(hubs, 0), obase(hubs, 0);
     std::vector<int> ctb(hubs, 0);
+    int mids = 0;
     long long offm_total = 0, ct_total = 0, ct_sr_start = 0;
     int max_tasks = 0, max_multi = 0;
     for (int sr = 0; sr < S; ++sr) {
@@ -1186,10 +1203,15 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         task_base_[sr] = (int)tasks.size();
         int c = 0, t = 0;
         (void)0;
+        mid_base_[sr] = mids;
         for (int i = 0; i < hub_n_[sr]; ++i) {
             const int d = deg[hub_base_[sr] + i];
-            const int nch = (d + kHubChunk - 1) / kHubChunk;
             prefix.push_back(c);
+            if (d <= kMidChunk) {  // the mid-size kernel, no chunk of the big one
+                midpos[hub_base_[sr] + i] = mids++ - mid_base_[sr];
+                continue;
+            }
+            const int nch = (d + kHubChunk - 1) / kHubChunk;
             c += nch;
             if (nch > 1) {
                 const int P = std::min(kBucketsPerChunk * nch, kMaxBuckets);
@@ -1214,6 +1236,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         ct_total += 1;  // zero sentinel after the sub-round's counts
         ct_sr_start = ct_total;
         hub_chunks_[sr] = c;
+        mid_n_[sr] = mids - mid_base_[sr];
         hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
         hub_tasks_[sr] = t;
         max_tasks = std::max(max_tasks, t);
@@ -1262,6 +1285,17 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         KH_CHECK(hipMemcpyAsync(multi_.p, multi.data(), sizeof(int) * multi.size(),
                                 hipMemcpyHostToDevice, s));
     build_chunk_maps(s);
+    mmap_.reserve(std::max(mids, 1));
+    midpos_.reserve(hubs);
+    KH_CHECK(hipMemcpyAsync(midpos_.p, midpos.data(), sizeof(int) * hubs, hipMemcpyHostToDevice,
+                            s));
+    for (int sr = 0; sr < S; ++sr) {
+        KH_REQUIRE(mid_base_[sr] + mid_n_[sr] <= std::max(mids, 1), "mid hub map overflow");
+        if (mid_n_[sr] > 0)
+            hipLaunchKernelGGL(build_mid_map, dim3(grid_for(hub_n_[sr], 256)), dim3(256), 0, s,
+                               lists_.p + host_offsets[sr * kNumBins + kNumBins - 1], hub_n_[sr],
+                               midpos_.p + hub_base_[sr], mmap_.p + mid_base_[sr]);
+    }
     {
         long long all_tasks = 0;  // one descriptor per (hub, bucket) task of every sub-round
         for (int sr = 0; sr < S; ++sr) all_tasks += hub_tasks_[sr];
@@ -1425,8 +1459,15 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         const long long* to = tab_off_.p + hub_base_[sr];
                         const long long* ob = offm_base_.p + hub_base_[sr];
                         const int* ctb = ct_base_.p + hub_base_[sr];
-                        if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
-                        else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        if (hub_chunks_[sr] > 0) {
+                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        }
+                        if (mid_n_[sr] > 0) {
+                            const int4* mm = mmap_.p + mid_base_[sr];
+                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                        }
                         ps.close();  // the hub scope times propose_hub_chunk alone
                         if (hub_multi_n_[sr] > 0) {
                             ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);

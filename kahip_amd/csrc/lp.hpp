@@ -82,6 +82,8 @@ private:
     int chunk_base_[kMaxSub] = {};  // offset into chunk_prefix_
     int cmap_base_[kMaxSub] = {};   // offset into cmap_ (chunks of the earlier sub-rounds)
     int tmap_base_[kMaxSub] = {};   // offset into tmap_ (tasks of the earlier sub-rounds)
+    int mid_n_[kMaxSub] = {};       // hubs of the mid-size kernel per sub-round
+    int mid_base_[kMaxSub] = {};    // offset into mmap_
     int multi_base_[kMaxSub] = {};  // offset into multi_
     int task_base_[kMaxSub] = {};   // offset into task_prefix_
     int hub_tasks_[kMaxSub] = {};   // (hub, bucket) tasks per sub-round
@@ -96,7 +98,8 @@ private:
     DevBuf<long long> offm_base_;
     DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_;
     DevBuf<int4> bres_;
-    DevBuf<int4> cmap_, tmap_;  // hub-chunk and (hub, bucket) task descriptors
+    DevBuf<int4> cmap_, tmap_, mmap_;  // hub-chunk, (hub, bucket) task, mid-size hub descriptors
+    DevBuf<int> midpos_;
     DevBuf<uint8_t> fb_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
     DevBuf<unsigned long long> edges_;
