@@ -13,7 +13,8 @@
 //   count       coarse weights, member counts, summed member degrees (row capacity bound).
 //   rows_lds      one wave per coarse node: LDS hash (q -> w, min e) of <= 512 entries, then an
 //                 LDS bitonic sort by min e; rows with more distinct neighbours go to
-//   rows_lds_big  one 1024-thread workgroup per row, 128 KB of LDS, <= 4096 distinct; beyond that
+//   rows_lds_big  one workgroup per row: 256 threads and 32 KB of LDS for <= 1024 distinct, then
+//                 1024 threads and 128 KB of LDS for <= 4096 distinct; beyond that
 //   rows_global_* the row's member edges in chunks of 4096 over many workgroups, LDS
 //                 pre-aggregation, one global table per row, then a segmented radix sort by min e.
 //   compact     rows are gathered into the final CSR after a scan of the coarse degrees.
@@ -39,7 +40,8 @@ constexpr int kDirectGlobal = 4096;  // rows with more member edges go to the ch
 constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
 constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
 constexpr int kBigMaxDistinct = 4096;
-constexpr int kHeavyList = 512;    // members whose edges are shared by the whole workgroup
+constexpr int kMidThreads = 256;   // between them: one 256-thread workgroup per row,
+constexpr int kMidMaxDistinct = 1024;  // 24 KB of hash and 8 KB of sort keys (5 per CU)
 constexpr int kGChunk = 4096;      // member edges per workgroup in the global tier
 constexpr int kGCap = 8192;        // LDS hash slots of a global-tier chunk
 constexpr int kGThreads = 512;
@@ -259,59 +261,37 @@ __global__ __launch_bounds__(64) void rows_lds(
     if (lane == 0) cdeg[p] = cnt;
 }
 
-// Visits every fine edge of the members mb..me-1 of one coarse node with a whole workgroup:
-// members of degree <= kHeavy are taken one per thread; heavier members are queued in LDS and
-// their edges are then shared by all threads. f(e) is called once per edge; stop() lets the
-// caller abandon the row early.
-template <int THREADS, class F, class Stop>
-__device__ inline void for_member_edges(const int* __restrict__ members, int mb, int me,
-                                        const int* __restrict__ xadj, int* heavy, int* n_heavy,
-                                        F&& f, Stop&& stop) {
-    constexpr int kHeavy = 64;
-    if (threadIdx.x == 0) *n_heavy = 0;
-    __syncthreads();
-    for (int mi = mb + threadIdx.x; mi < me; mi += THREADS) {
-        if (stop()) break;
-        const int u = members[mi];
-        const int b = xadj[u], d = xadj[u + 1] - b;
-        if (d > kHeavy) {
-            const int slot = atomicAdd(n_heavy, 1);
-            if (slot < kHeavyList) {
-                heavy[slot] = u;
-                continue;
-            }
-        }
-        for (int e = b; e < b + d; ++e) f(e);
-    }
-    __syncthreads();
-    const int nh = min(*n_heavy, kHeavyList);
-    for (int h = 0; h < nh; ++h) {
-        if (stop()) break;
-        const int u = heavy[h];
-        for (int e = xadj[u] + threadIdx.x; e < xadj[u + 1]; e += THREADS) f(e);
-    }
-    __syncthreads();
-}
-
-// Rows with more than kRowMaxDistinct distinct neighbours: one workgroup of kBigThreads per row,
-// LDS hash of kBigCap slots and an LDS bitonic sort of up to kBigMaxDistinct entries; rows with
-// even more distinct neighbours are passed on to the global tier.
-__global__ __launch_bounds__(kBigThreads) void rows_lds_big(
+// Rows with more than kRowMaxDistinct distinct neighbours: one workgroup of THREADS per row
+// (grid-stride over the list), an LDS hash of up to CAP slots and an LDS bitonic sort of up to
+// MAXD entries; rows with more distinct neighbours are passed on to the next tier. The members'
+// edges are expanded with a load-balanced scan, MAXD members at a time: S holds each member's
+// (inclusive degree prefix, row start), every thread takes every THREADS-th edge of the batch
+// and gathers kEpl targets at a time, so the target and cluster loads of many edges are in
+// flight together. After the hash phase S holds the sort keys.
+template <int THREADS, int CAP, int MAXD>
+__global__ __launch_bounds__(THREADS) void rows_lds_big(
     const int* __restrict__ list, const int* __restrict__ list_count, const int* __restrict__ xadj,
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
-    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ tpos,
-    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw,
-    int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
-    __shared__ int K[kBigCap];
-    __shared__ int V[kBigCap];
-    __shared__ int M[kBigCap];
-    __shared__ unsigned long long S[kBigMaxDistinct];
-    __shared__ int heavy[kHeavyList];
-    __shared__ int n_heavy, used, over, nkeys;
+    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
+    const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
+    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    static_assert(CAP == 2 * MAXD && MAXD % THREADS == 0, "table shape");
+    constexpr int kEpl = 4;
+    constexpr int R = MAXD / THREADS;  // members per thread in the prefix scan
+    __shared__ int K[CAP];
+    __shared__ int V[CAP];
+    __shared__ int M[CAP];
+    __shared__ unsigned long long S[MAXD];
+    __shared__ int wsum[THREADS / 64];
+    __shared__ int used, over, nkeys;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int count = *list_count;
     for (int li = blockIdx.x; li < count; li += gridDim.x) {
         const int p = list[li];
-        for (int i = threadIdx.x; i < kBigCap; i += kBigThreads) {
+        const int fd = fdeg[p];
+        int cap = 64;  // distinct <= fd
+        while (cap < 2 * fd && cap < CAP) cap <<= 1;
+        for (int i = threadIdx.x; i < cap; i += THREADS) {
             K[i] = -1;
             V[i] = 0;
             M[i] = 0x7fffffff;
@@ -322,31 +302,94 @@ __global__ __launch_bounds__(kBigThreads) void rows_lds_big(
             nkeys = 0;
         }
         __syncthreads();
-        for_member_edges<kBigThreads>(
-            members, moff[p], moff[p + 1], xadj, heavy, &n_heavy,
-            [&](int e) {
-                if (over) return;
-                const int q = map[adjncy[e]];
-                if (q == p) return;
-                uint32_t h = mix32((uint32_t)q) & (uint32_t)(kBigCap - 1);
-                while (true) {
-                    const int old = atomicCAS(&K[h], -1, q);
-                    if (old == -1 && atomicAdd(&used, 1) >= kBigMaxDistinct) over = 1;
-                    if (old == -1 || old == q) {
-                        atomicAdd(&V[h], adjwgt[e]);
-                        atomicMin(&M[h], e);
-                        break;
-                    }
-                    h = (h + 1) & (uint32_t)(kBigCap - 1);
+        const int mb = moff[p], me = moff[p + 1];
+        for (int base = mb; base < me; base += MAXD) {
+            // (inclusive degree prefix, row start) of the batch's members
+            const int nb = min(MAXD, me - base);
+            int st[R], dg[R], run = 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int i = threadIdx.x * R + j;
+                st[j] = 0;
+                dg[j] = 0;
+                if (i < nb) {
+                    const int u = members[base + i];
+                    st[j] = xadj[u];
+                    dg[j] = xadj[u + 1] - st[j];
                 }
-            },
-            [&]() { return over != 0; });
+                run += dg[j];
+            }
+            int incl = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) wsum[wave] = incl;
+            __syncthreads();
+            int acc = incl - run, total = 0;
+            for (int w = 0; w < THREADS / 64; ++w) {
+                if (w < wave) acc += wsum[w];
+                total += wsum[w];
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                acc += dg[j];
+                S[threadIdx.x * R + j] = ((unsigned long long)(uint32_t)acc << 32) | (uint32_t)st[j];
+            }
+            __syncthreads();
+            for (int t0 = 0; t0 < total; t0 += THREADS * kEpl) {
+                if (over) break;  // no barrier in this loop
+                int es[kEpl], qs[kEpl], ws[kEpl];
+#pragma unroll
+                for (int j = 0; j < kEpl; ++j) {
+                    const int t = t0 + threadIdx.x + THREADS * j;
+                    es[j] = -1;
+                    if (t < total) {
+                        int lo = 0, hi = nb - 1;  // first member whose inclusive prefix exceeds t
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if ((int)(S[mid] >> 32) > t) hi = mid; else lo = mid + 1;
+                        }
+                        const int before = lo ? (int)(S[lo - 1] >> 32) : 0;
+                        es[j] = (int)(uint32_t)S[lo] + (t - before);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kEpl; ++j) {
+                    qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
+                    ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+                }
+#pragma unroll
+                for (int j = 0; j < kEpl; ++j)
+                    if (es[j] >= 0) qs[j] = map[qs[j]];
+#pragma unroll
+                for (int j = 0; j < kEpl; ++j) {
+                    const int q = qs[j];
+                    if (es[j] < 0 || q == p) continue;
+                    if (over) break;
+                    uint32_t h = mix32((uint32_t)q) & (uint32_t)(cap - 1);
+                    while (true) {
+                        const int old = atomicCAS(&K[h], -1, q);
+                        if (old == -1 && atomicAdd(&used, 1) >= MAXD) over = 1;
+                        if (old == -1 || old == q) {
+                            atomicAdd(&V[h], ws[j]);
+                            atomicMin(&M[h], es[j]);
+                            break;
+                        }
+                        h = (h + 1) & (uint32_t)(cap - 1);
+                    }
+                }
+            }
+            __syncthreads();  // S is rewritten by the next batch; over is read below
+            if (over) break;  // uniform after the barrier
+        }
         if (over) {
             if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
             __syncthreads();
             continue;
         }
-        for (int i = threadIdx.x; i < kBigCap; i += kBigThreads) {
+        for (int i = threadIdx.x; i < cap; i += THREADS) {
             if (K[i] >= 0) {
                 const int idx = atomicAdd(&nkeys, 1);
                 S[idx] = ((unsigned long long)(uint32_t)M[i] << 32) | (uint32_t)i;
@@ -356,11 +399,11 @@ __global__ __launch_bounds__(kBigThreads) void rows_lds_big(
         const int cnt = nkeys;
         int len = 1;
         while (len < cnt) len <<= 1;
-        for (int i = cnt + threadIdx.x; i < len; i += kBigThreads) S[i] = ~0ull;
+        for (int i = cnt + threadIdx.x; i < len; i += THREADS) S[i] = ~0ull;
         __syncthreads();
         for (int k = 2; k <= len; k <<= 1) {
             for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = threadIdx.x; i < len; i += kBigThreads) {
+                for (int i = threadIdx.x; i < len; i += THREADS) {
                     const int l = i ^ j;
                     if (l > i) {
                         const unsigned long long a = S[i], b = S[l];
@@ -374,7 +417,7 @@ __global__ __launch_bounds__(kBigThreads) void rows_lds_big(
             }
         }
         const int off = tpos[p];
-        for (int i = threadIdx.x; i < cnt; i += kBigThreads) {
+        for (int i = threadIdx.x; i < cnt; i += THREADS) {
             const int slot = (int)(S[i] & 0xffffffffu);
             tq[off + i] = K[slot];
             tw[off + i] = V[slot];
@@ -829,9 +872,10 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     tq_.reserve(m2);
     tw_.reserve(m2);
     ovl_list_.reserve(nc);
-    ovl_count_.reserve(2);
+    ovl_count_.reserve(3);
     ovl2_list_.reserve(nc);
-    fill_async(ovl_count_.p, 2, 0, s);
+    ovl3_list_.reserve(nc);
+    fill_async(ovl_count_.p, 3, 0, s);
     if (nc > 0) {
         // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
@@ -858,10 +902,17 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
                                ovl_count_.p);
     }
     if (nc > 0) {
+        // overflow of rows_lds -> mid tier (<= kMidMaxDistinct) -> big tier (<= kBigMaxDistinct)
+        // -> global tier (ovl2 list, which also holds the rows of more than kDirectGlobal edges)
         ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);
-        hipLaunchKernelGGL(rows_lds_big, dim3(256), dim3(kBigThreads), 0, s, ovl_list_.p,
-                           ovl_count_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
-                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1);
+        hipLaunchKernelGGL((rows_lds_big<kMidThreads, 2 * kMidMaxDistinct, kMidMaxDistinct>),
+                           dim3(256 * 5), dim3(kMidThreads), 0, s, ovl_list_.p, ovl_count_.p,
+                           g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p,
+                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl3_list_.p, ovl_count_.p + 2);
+        hipLaunchKernelGGL((rows_lds_big<kBigThreads, kBigCap, kBigMaxDistinct>), dim3(256),
+                           dim3(kBigThreads), 0, s, ovl3_list_.p, ovl_count_.p + 2, g.xadj,
+                           g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p, tpos_.p,
+                           cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1);
     }
     int novl = 0;
     KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));

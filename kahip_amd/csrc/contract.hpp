@@ -37,7 +37,7 @@ private:
     DevBuf<char> scan_tmp_;
     DevBuf<int> leader_, flag_, newid_;
     DevBuf<int> cnt_, fdeg_, moff_, tpos_, cursor_, cdeg_, members_, tq_, tw_;
-    DevBuf<int> ovl_list_, ovl2_list_, ovl_count_, obound_, obase_, oend_;
+    DevBuf<int> ovl_list_, ovl2_list_, ovl3_list_, ovl_count_, obound_, obase_, oend_;
     DevBuf<int> rsmall_, rmid_, rcount_;
     DevBuf<long long> hcap_, hoff_;
     DevBuf<int> gchunk_prefix_, mdeg_, mpre_, ofd_;
