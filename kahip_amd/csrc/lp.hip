@@ -33,6 +33,7 @@ constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => l
 static_assert(kWaveCap == 1024, "propose_wave loads up to kWaveCap / 2 edges per wave");
 constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
 constexpr int kBlock = 256;
+constexpr int kFrontierGrid = 2048;  // workgroups of a refinement list kernel (grid-stride)
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
@@ -151,10 +152,15 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
 }
 
 // ---------------------------------------------------------------- G lanes per node (deg <= G)
+// dcount: the list's length on the device (the compacted frontier of a refinement round);
+// workgroups then stride over the list's virtual blocks, so a small frontier costs few of them.
 template <LpMode M, int G>
 __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
-                                                        int count) {
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
+                                                        int count, const int* __restrict__ dcount) {
+    if (dcount) count = *dcount;
+    const int nvb = (int)(((long long)count * G + kBlock - 1) / kBlock);
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const int tid = vb * kBlock + threadIdx.x;
     const int gi = tid / G;
     const int gl = threadIdx.x & (G - 1);
     bool valid = gi < count;
@@ -238,7 +244,9 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
         __syncthreads();
         if (valid && gl == 0) atomicAdd(&tsum, (unsigned long long)deg);
         __syncthreads();
-        if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[blockIdx.x & 63], tsum);
+        if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[vb & 63], tsum);
+    }
+    __syncthreads();  // the LDS of this virtual block is read before the next one writes it
     }
 }
 
@@ -247,12 +255,16 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
 // dense counter per label when the labels are the k blocks of the refinement and k is small.
 template <LpMode M, bool DENSE>
 __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
-                                                       int count, int num_labels) {
+                                                       int count, int num_labels,
+                                                       const int* __restrict__ dcount) {
     __shared__ int skey[kBlock / 64][DENSE ? 1 : kWaveCap];
     __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : kWaveCap];
+    if (dcount) count = *dcount;
+    const int nvb = (count + kBlock / 64 - 1) / (kBlock / 64);
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int gi = blockIdx.x * (kBlock / 64) + wave;
+    const int gi = vb * (kBlock / 64) + wave;
     bool valid = gi < count;
     const int v = valid ? list[gi] : 0;
     if (M == LpMode::Refine && valid) valid = a.active[v] != 0;
@@ -327,7 +339,9 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     reduce_cand<64>(c);
     if (valid && lane == 0) {
         decide<M>(a, v, own, nw, conn_own, c);
-        if (M == LpMode::Refine) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)deg);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[vb & 63], (unsigned long long)deg);
+    }
+    __syncthreads();  // the LDS of this virtual block is read before the next one clears it
     }
 }
 
@@ -956,6 +970,43 @@ __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, 
     if (l >= 0) lists[offsets[l] + base[l] + r] = v;
 }
 
+// Frontier of a refinement round: the active nodes of every non-hub list, compacted per list
+// into clists (same offsets as lists); ccount[l] receives list l's length. The order inside a
+// list is free: the synchronous schedule does not depend on it.
+__global__ __launch_bounds__(256) void compact_active(const int* __restrict__ lists,
+                                                      const int* __restrict__ offsets, int nseg,
+                                                      int total, const uint8_t* __restrict__ active,
+                                                      int* __restrict__ ccount,
+                                                      int* __restrict__ clists) {
+    __shared__ int off[kMaxSub * kNumBins];
+    __shared__ int local[kMaxSub * kNumBins];
+    __shared__ int base[kMaxSub * kNumBins];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) {
+        off[i] = offsets[i];
+        local[i] = 0;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int l = -1, r = 0, v = 0;
+    if (i < total) {
+        int lo = 0, hi = nseg - 1;  // last list with off <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        v = lists[i];
+        if (lo % kNumBins != kNumBins - 1 && active[v]) {
+            l = lo;
+            r = atomicAdd(&local[l], 1);
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nseg; j += blockDim.x)
+        base[j] = local[j] ? atomicAdd(&ccount[j], local[j]) : 0;
+    __syncthreads();
+    if (l >= 0) clists[off[l] + base[l] + r] = v;
+}
+
 // Block descriptors of the hub kernels, built once per level: chunk c of the sub-round's hub
 // list -> {hub index, chunk of the hub, chunks of the hub, node}.
 __global__ void build_chunk_map(const int* __restrict__ list, int count,
@@ -1068,6 +1119,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     dense_hub_ = mode == LpMode::Refine && num_labels <= kHubDense;
     const int L = kMaxSub * kNumBins;
     lists_.reserve(g.n);
+    if (mode == LpMode::Refine) {
+        clists_.reserve(g.n);
+        ccount_.reserve(L);
+    }
     counts_.reserve(L);
     cursor_.reserve(L);
     offsets_.reserve(L);
@@ -1399,11 +1454,19 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         }
     }
     const int nl = num_labels_;
+    const int nseg = S * kNumBins;
+    const int listed = host_offsets[nseg - 1] + host_counts[nseg - 1];
     for (int r = 0; r < rounds; ++r) {
         a.tie_seed = tie_seed_of(seed, r);
         a.moves = moves_.p + r;
         a.active = cl ? nullptr : act;
         a.next_active = cl ? nullptr : nxt;
+        if (!cl && listed > 0) {
+            ProfScope ps(prof, base + "_frontier", 0.0, 0, s);
+            fill_async(ccount_.p, nseg, 0, s);
+            hipLaunchKernelGGL(compact_active, dim3(grid_for(listed, 256)), dim3(256), 0, s,
+                               lists_.p, offsets_.p, nseg, listed, act, ccount_.p, clists_.p);
+        }
         for (int sr = 0; sr < S; ++sr) {
             a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
@@ -1412,32 +1475,35 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 const int cnt = host_counts[l];
                 sub_total += cnt;
                 if (cnt == 0) continue;
-                const int* lst = lists_.p + host_offsets[l];
+                // refinement: the round's frontier of the list (length on the device)
+                const bool fr = !cl && b < kNumBins - 1;
+                const int* lst = (fr ? clists_.p : lists_.p) + host_offsets[l];
+                const int* dc = fr ? ccount_.p + l : nullptr;
                 // algorithmic bytes: 12 B per traversed edge (target, weight, label gather) and
                 // 16 B per node (two offsets, weight, own label); label weights are O(moved).
                 const double bytes = cl ? 12.0 * (double)host_edges[l] + 16.0 * cnt : 0.0;
                 ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? host_edges[l] : 0, s);
                 switch (b) {
                     case 0:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(std::min(grid_for((long long)cnt * 8, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 1:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(std::min(grid_for((long long)cnt * 16, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 2:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(std::min(grid_for((long long)cnt * 32, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 3:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(std::min(grid_for((long long)cnt * 64, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 4:
-                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
-                        else if (dense_wave_) hipLaunchKernelGGL((propose_wave<LpMode::Refine, true>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
-                        else hipLaunchKernelGGL((propose_wave<LpMode::Refine, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl);
+                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl, nullptr);
+                        else if (dense_wave_) hipLaunchKernelGGL((propose_wave<LpMode::Refine, true>), dim3(std::min(grid_for(cnt, kBlock / 64), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, nl, dc);
+                        else hipLaunchKernelGGL((propose_wave<LpMode::Refine, false>), dim3(std::min(grid_for(cnt, kBlock / 64), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, nl, dc);
                         break;
                     default: {
                         if (!cl && dense_hub_) {

@@ -102,6 +102,7 @@ private:
     DevBuf<int> midpos_;
     DevBuf<uint8_t> fb_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
+    DevBuf<int> clists_, ccount_;  // refinement frontier per list (compacted each round)
     DevBuf<unsigned long long> edges_;
     DevBuf<int> prop_t_, prop_gain_, moves_;
     DevBuf<uint8_t> prop_ok_;
