@@ -32,6 +32,9 @@ namespace {
 constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => load <= 1/2)
 static_assert(kWaveCap == 1024, "propose_wave loads up to kWaveCap / 2 edges per wave");
 constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
+// Clustering splits the wave list: nodes of degree <= kWaveSmall come first and use a kernel of
+// 256 slots per wave (8 KB of LDS per workgroup instead of 32 KB, so more waves per CU).
+constexpr int kWaveSmall = 128;
 constexpr int kBlock = 256;
 constexpr int kFrontierGrid = 2048;  // workgroups of a refinement list kernel (grid-stride)
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
@@ -253,12 +256,13 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
 // ---------------------------------------------------------------- one wave per node
 // Labels are aggregated in LDS: a hash table (open addressing, LDS atomics) in general, or a
 // dense counter per label when the labels are the k blocks of the refinement and k is small.
-template <LpMode M, bool DENSE>
+template <LpMode M, bool DENSE, int WCAP = kWaveCap>
 __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
                                                        int count, int num_labels,
                                                        const int* __restrict__ dcount) {
-    __shared__ int skey[kBlock / 64][DENSE ? 1 : kWaveCap];
-    __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : kWaveCap];
+    static_assert(!DENSE || WCAP == kWaveCap, "the dense variant serves every wave degree");
+    __shared__ int skey[kBlock / 64][DENSE ? 1 : WCAP];
+    __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : WCAP];
     if (dcount) count = *dcount;
     const int nvb = (count + kBlock / 64 - 1) / (kBlock / 64);
     for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     if (DENSE) {
         cap = num_labels;
     } else {
-        while (cap < 2 * deg) cap <<= 1;  // deg <= 512 => cap <= kWaveCap
+        while (cap < 2 * deg) cap <<= 1;  // deg <= WCAP / 2 => cap <= WCAP
     }
     int* K = skey[wave];
     int* V = sval[wave];
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     __syncthreads();
     // All of a lane's (<= 8) edges are loaded first so that their label gathers are in flight
     // together; the LDS insertions follow.
-    constexpr int EPL = kWaveCap / 2 / 64;  // edges per lane at the maximum degree (512)
+    constexpr int EPL = WCAP / 2 / 64;  // edges per lane at the maximum degree (WCAP / 2)
     int labs[EPL], ws[EPL];
 #pragma unroll
     for (int j = 0; j < EPL; ++j) {
@@ -918,23 +922,29 @@ __device__ inline int bin_of(const BinConfig& cfg, int deg) {
     return kNumBins - 1;
 }
 
+// counts[l] / edges[l]: nodes and edges of list l = sub-round * kNumBins + bin; with split,
+// counts[kMaxSub * kNumBins + sr] also receives the wave-bin nodes of degree <= kWaveSmall.
 __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, int S,
-                          BinConfig cfg, const uint8_t* __restrict__ include,
+                          BinConfig cfg, const uint8_t* __restrict__ include, int split,
                           int* __restrict__ counts, unsigned long long* __restrict__ edges) {
-    __shared__ int local[kMaxSub * kNumBins];
-    __shared__ unsigned long long ledges[kMaxSub * kNumBins];
-    for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) {
+    constexpr int L = kMaxSub * kNumBins;
+    __shared__ int local[L + kMaxSub];
+    __shared__ unsigned long long ledges[L];
+    for (int i = threadIdx.x; i < L + kMaxSub; i += blockDim.x) {
         local[i] = 0;
-        ledges[i] = 0;
+        if (i < L) ledges[i] = 0;
     }
     __syncthreads();
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n && (!include || include[v])) {
         const int deg = xadj[v + 1] - xadj[v];
         if (deg > 0) {
-            const int l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
+            const int sr = subround_of(seed, v, S);
+            const int b = bin_of(cfg, deg);
+            const int l = sr * kNumBins + b;
             atomicAdd(&local[l], 1);
             atomicAdd(&ledges[l], (unsigned long long)deg);
+            if (split && b == kNumBins - 2 && deg <= kWaveSmall) atomicAdd(&local[L + sr], 1);
         }
     }
     __syncthreads();
@@ -944,30 +954,42 @@ __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, in
             atomicAdd(&edges[i], ledges[i]);
         }
     }
+    for (int i = threadIdx.x; i < S; i += blockDim.x)
+        if (local[L + i]) atomicAdd(&counts[L + i], local[L + i]);
 }
 
+// With split, a wave list holds its nodes of degree <= kWaveSmall first (counts[L + sr] of
+// them, cursor slot L + sr), then the others.
 __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, int S,
-                            BinConfig cfg, const uint8_t* __restrict__ include,
-                            const int* __restrict__ offsets, int* __restrict__ cursor,
-                            int* __restrict__ lists) {
-    __shared__ int local[kMaxSub * kNumBins];
-    __shared__ int base[kMaxSub * kNumBins];
-    for (int i = threadIdx.x; i < kMaxSub * kNumBins; i += blockDim.x) local[i] = 0;
+                            BinConfig cfg, const uint8_t* __restrict__ include, int split,
+                            const int* __restrict__ counts, const int* __restrict__ offsets,
+                            int* __restrict__ cursor, int* __restrict__ lists) {
+    constexpr int L = kMaxSub * kNumBins;
+    __shared__ int local[L + kMaxSub];
+    __shared__ int base[L + kMaxSub];
+    for (int i = threadIdx.x; i < L + kMaxSub; i += blockDim.x) local[i] = 0;
     __syncthreads();
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    int l = -1, r = 0;
+    int l = -1, slot = -1, r = 0, skip = 0;
     if (v < n && (!include || include[v])) {
         const int deg = xadj[v + 1] - xadj[v];
         if (deg > 0) {
-            l = subround_of(seed, v, S) * kNumBins + bin_of(cfg, deg);
-            r = atomicAdd(&local[l], 1);
+            const int sr = subround_of(seed, v, S);
+            const int b = bin_of(cfg, deg);
+            l = sr * kNumBins + b;
+            slot = l;
+            if (split && b == kNumBins - 2) {
+                if (deg <= kWaveSmall) slot = L + sr;  // front part of the wave list
+                else skip = counts[L + sr];            // after the small ones
+            }
+            r = atomicAdd(&local[slot], 1);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < S * kNumBins; i += blockDim.x)
+    for (int i = threadIdx.x; i < L + kMaxSub; i += blockDim.x)
         base[i] = local[i] ? atomicAdd(&cursor[i], local[i]) : 0;
     __syncthreads();
-    if (l >= 0) lists[offsets[l] + base[l] + r] = v;
+    if (l >= 0) lists[offsets[l] + skip + base[slot] + r] = v;
 }
 
 // Frontier of a refinement round: the active nodes of every non-hub list, compacted per list
@@ -1123,8 +1145,8 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         clists_.reserve(g.n);
         ccount_.reserve(L);
     }
-    counts_.reserve(L);
-    cursor_.reserve(L);
+    counts_.reserve(L + kMaxSub);
+    cursor_.reserve(L + kMaxSub);
     offsets_.reserve(L);
     edges_.reserve(L);
     prop_t_.reserve(g.n);
@@ -1142,8 +1164,9 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     trav_.reserve(64);
 
-    fill_async(counts_.p, L, 0, s);
-    fill_async(cursor_.p, L, 0, s);
+    fill_async(counts_.p, L + kMaxSub, 0, s);
+    fill_async(cursor_.p, L + kMaxSub, 0, s);
+    const int split = mode == LpMode::Cluster ? 1 : 0;  // small-degree wave kernel (clustering)
     fill_async(edges_.p, L, 0ull, s);
     fill_async(prop_t_.p, g.n, -1, s);
     fill_async(inc_.p, num_labels, 0, s);
@@ -1152,8 +1175,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     fill_async(dtop_.p, num_labels, 0ull, s);
     if (g.n > 0)
         hipLaunchKernelGGL(bin_count, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
-                           seed, S, bins, include, counts_.p, edges_.p);
+                           seed, S, bins, include, split, counts_.p, edges_.p);
     KH_CHECK(hipMemcpyAsync(host_counts, counts_.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipMemcpyAsync(host_wave_small_, counts_.p + L, sizeof(int) * kMaxSub,
+                            hipMemcpyDeviceToHost, s));
     KH_CHECK(hipMemcpyAsync(host_edges, edges_.p, sizeof(long long) * L, hipMemcpyDeviceToHost,
                             s));
     KH_CHECK(hipStreamSynchronize(s));
@@ -1165,7 +1190,8 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     KH_CHECK(hipMemcpyAsync(offsets_.p, host_offsets, sizeof(int) * L, hipMemcpyHostToDevice, s));
     if (g.n > 0)
         hipLaunchKernelGGL(bin_scatter, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
-                           seed, S, bins, include, offsets_.p, cursor_.p, lists_.p);
+                           seed, S, bins, include, split, counts_.p, offsets_.p, cursor_.p,
+                           lists_.p);
 
     // hubs (last bin): chunk prefix per sub-round list, global tables for multi-chunk hubs
     int hubs = 0;
@@ -1501,7 +1527,11 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                         else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(std::min(grid_for((long long)cnt * 64, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 4:
-                        if (cl) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt, kBlock / 64)), dim3(kBlock), 0, s, a, lst, cnt, nl, nullptr);
+                        if (cl) {
+                            const int ns = host_wave_small_[sr];  // degree <= kWaveSmall, first
+                            if (ns > 0) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false, 2 * kWaveSmall>), dim3(grid_for(ns, kBlock / 64)), dim3(kBlock), 0, s, a, lst, ns, nl, nullptr);
+                            if (cnt > ns) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt - ns, kBlock / 64)), dim3(kBlock), 0, s, a, lst + ns, cnt - ns, nl, nullptr);
+                        }
                         else if (dense_wave_) hipLaunchKernelGGL((propose_wave<LpMode::Refine, true>), dim3(std::min(grid_for(cnt, kBlock / 64), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, nl, dc);
                         else hipLaunchKernelGGL((propose_wave<LpMode::Refine, false>), dim3(std::min(grid_for(cnt, kBlock / 64), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, nl, dc);
                         break;

@@ -66,6 +66,7 @@ public:
     long long traversed(hipStream_t s);
 
     int host_counts[kMaxSub * kNumBins] = {};
+    int host_wave_small_[kMaxSub] = {};  // clustering: wave-list nodes of the small-LDS kernel
     long long host_edges[kMaxSub * kNumBins] = {};
     int host_offsets[kMaxSub * kNumBins] = {};
 
