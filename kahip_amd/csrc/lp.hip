@@ -658,8 +658,8 @@ __global__ __launch_bounds__(256) void hub_scatter(
 template <LpMode M, bool SMALL>
 __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
     LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
-    int n_tasks, const int4* __restrict__ tmap, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ ct_base, const int* __restrict__ posT,
+    int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
+    const int* __restrict__ chunk_prefix, const int* __restrict__ ct_base, const int* __restrict__ posT,
     const int2* __restrict__ bucketed, const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
     int* __restrict__ bown, uint8_t* __restrict__ fb, int* __restrict__ big_tasks,
@@ -671,19 +671,19 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
     const int n_iter = SMALL ? n_tasks : *n_big_tasks;
     for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
         const int task = SMALL ? it : big_tasks[it];
-        const int4 tm = tmap[task];  // {multi-chunk hub index, bucket, buckets of the hub, hub index}
-        const int mi = tm.x;
-        const int gi = tm.w;
-        const int v = list[gi];
+        const int4 tm = tmap[task];  // {first count index, one past the last, hub index, node}
+        const int2 ts = tsub[task];  // {multi-chunk hub index, bucket | buckets << 16}
+        const int mi = ts.x;
+        const int gi = tm.z;
+        const int v = tm.w;
         if (M == LpMode::Refine && a.active[v] == 0) {  // uniform per block
             __syncthreads();
             continue;
         }
-        const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
-        const int P = tm.z;
-        const int b = tm.y;
-        const int r0 = posT[ct_base[gi] + b * nch];
-        const int r1 = posT[ct_base[gi] + (b + 1) * nch];
+        const int P = ts.y >> 16;
+        const int b = ts.y & 0xffff;
+        const int r0 = posT[tm.x];
+        const int r1 = posT[tm.y];
         const int total = r1 - r0;
         if (SMALL && total > kSmallMax) {
             if (threadIdx.x == 0) big_tasks[atomicAdd(n_big_tasks, 1)] = task;
@@ -1050,13 +1050,25 @@ __global__ void build_mid_map(const int* __restrict__ list, int count,
 }
 
 // (hub, bucket) task t of a sub-round -> {multi-chunk hub index, bucket, buckets, hub index}.
-__global__ void build_task_map(const int* __restrict__ multi, int n_multi,
-                               const int* __restrict__ task_prefix, int4* __restrict__ tmap) {
+// tmap[t] = {index of the bucket's first count in the bucket-major table, index one past its
+// last, hub index, node}; tsub[t] = {multi-chunk hub index, bucket | buckets << 16}.
+__global__ void build_task_map(const int* __restrict__ list, const int* __restrict__ multi,
+                               int n_multi, const int* __restrict__ task_prefix,
+                               const int* __restrict__ chunk_prefix,
+                               const int* __restrict__ ct_base, int4* __restrict__ tmap,
+                               int2* __restrict__ tsub) {
     const int mi = blockIdx.x * blockDim.x + threadIdx.x;
     if (mi >= n_multi) return;
     const int t0 = task_prefix[mi], t1 = task_prefix[mi + 1];
     const int gi = multi[mi];
-    for (int t = t0; t < t1; ++t) tmap[t] = make_int4(mi, t - t0, t1 - t0, gi);
+    const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int cb = ct_base[gi];
+    const int v = list[gi];
+    for (int t = t0; t < t1; ++t) {
+        const int b = t - t0;
+        tmap[t] = make_int4(cb + b * nch, cb + (b + 1) * nch, gi, v);
+        tsub[t] = make_int2(mi, b | ((t1 - t0) << 16));
+    }
 }
 
 __global__ void hub_degrees(const int* __restrict__ list, int count, const int* __restrict__ xadj,
@@ -1381,13 +1393,17 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         long long all_tasks = 0;  // one descriptor per (hub, bucket) task of every sub-round
         for (int sr = 0; sr < S; ++sr) all_tasks += hub_tasks_[sr];
         tmap_.reserve(std::max<long long>(all_tasks, 1));
+        tsub_.reserve(std::max<long long>(all_tasks, 1));
         long long tb = 0;
         for (int sr = 0; sr < S; ++sr) {
             KH_REQUIRE(tb + hub_tasks_[sr] <= all_tasks, "task map overflow");
             if (hub_multi_n_[sr] > 0)
                 hipLaunchKernelGGL(build_task_map, dim3(grid_for(hub_multi_n_[sr], 256)), dim3(256),
-                                   0, s, multi_.p + multi_base_[sr], hub_multi_n_[sr],
-                                   task_prefix_.p + task_base_[sr], tmap_.p + tb);
+                                   0, s, lists_.p + host_offsets[sr * kNumBins + kNumBins - 1],
+                                   multi_.p + multi_base_[sr], hub_multi_n_[sr],
+                                   task_prefix_.p + task_base_[sr],
+                                   chunk_prefix_.p + chunk_base_[sr], ct_base_.p + hub_base_[sr],
+                                   tmap_.p + tb, tsub_.p + tb);
             tmap_base_[sr] = (int)tb;
             tb += hub_tasks_[sr];
         }
@@ -1570,6 +1586,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             const int* ml = multi_.p + multi_base_[sr];
                             const int* tp = task_prefix_.p + task_base_[sr];
                             const int4* tm = tmap_.p + tmap_base_[sr];
+                            const int2* tsb = tsub_.p + tmap_base_[sr];
                             const int nm = hub_multi_n_[sr];
                             Cand* br = reinterpret_cast<Cand*>(bres_.p);
                             const int nt = hub_tasks_[sr];
@@ -1581,12 +1598,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

@@ -101,6 +101,7 @@ private:
     DevBuf<int4> bres_;
     DevBuf<int4> cmap_, tmap_, mmap_;  // hub-chunk, (hub, bucket) task, mid-size hub descriptors
     DevBuf<int> midpos_;
+    DevBuf<int2> tsub_;  // (hub, bucket) task: {multi-chunk hub index, bucket | buckets << 16}
     DevBuf<uint8_t> fb_;
     DevBuf<int> lists_, counts_, cursor_, offsets_;
     DevBuf<int> clists_, ccount_;  // refinement frontier per list (compacted each round)
