@@ -203,43 +203,39 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
         }
     }
     reduce_cand<G>(c);
-    // The block's proposals are combined per (target, kind) in LDS before the atomics: many
-    // low-degree nodes propose the same (hub) label, and one atomic per distinct target keeps
-    // them from serialising on it.
+    // The block's proposals are combined per (target, kind) in an LDS hash before the atomics:
+    // many low-degree nodes propose the same (hub) label, and one atomic per distinct target
+    // keeps them from serialising on it.
     constexpr int NB = kBlock / G;
-    __shared__ int s_lab[NB];
-    __shared__ int s_w[NB];
-    __shared__ unsigned long long s_key[NB];
-    __shared__ uint8_t s_dep[NB];
-    const int slot = threadIdx.x / G;
-    if (gl == 0) {
-        const Contrib r = valid ? propose<M>(a, v, own, nw, conn_own, c)
-                                : Contrib{-1, 0, 0ull, false};
-        s_lab[slot] = r.label;
-        s_w[slot] = r.w;
-        s_key[slot] = r.key;
-        s_dep[slot] = r.departure ? 1 : 0;
+    constexpr int HC = 2 * NB;  // slots (load <= 1/2)
+    __shared__ int h_tag[HC];   // (label << 1) | departure, -1: empty
+    __shared__ int h_w[HC];
+    __shared__ unsigned long long h_key[HC];
+    if (threadIdx.x < HC) {
+        h_tag[threadIdx.x] = -1;
+        h_w[threadIdx.x] = 0;
+        h_key[threadIdx.x] = 0ull;
     }
+    const Contrib r = (valid && gl == 0) ? propose<M>(a, v, own, nw, conn_own, c)
+                                         : Contrib{-1, 0, 0ull, false};
     __syncthreads();
-    if (threadIdx.x < NB && s_lab[threadIdx.x] >= 0) {
-        const int i = threadIdx.x;
-        const int l = s_lab[i];
-        const uint8_t d = s_dep[i];
-        bool first = true;
-        for (int j = 0; j < i; ++j)
-            if (s_lab[j] == l && s_dep[j] == d) {
-                first = false;
+    if (r.label >= 0) {
+        const int tag = (r.label << 1) | (r.departure ? 1 : 0);
+        uint32_t h = mix32((uint32_t)tag) & (uint32_t)(HC - 1);
+        while (true) {
+            const int old = atomicCAS(&h_tag[h], -1, tag);
+            if (old == -1 || old == tag) {
+                atomicAdd(&h_w[h], r.w);
+                atomicMax(&h_key[h], r.key);
                 break;
             }
-        if (first) {
-            Contrib sum{l, 0, 0ull, d != 0};
-            for (int j = i; j < NB; ++j)
-                if (s_lab[j] == l && s_dep[j] == d) {
-                    sum.w += s_w[j];
-                    sum.key = s_key[j] > sum.key ? s_key[j] : sum.key;
-                }
-            add_contrib(a, sum);
+            h = (h + 1) & (uint32_t)(HC - 1);
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < HC && h_tag[threadIdx.x] >= 0) {
+        const int tag = h_tag[threadIdx.x];
+        add_contrib(a, Contrib{tag >> 1, h_w[threadIdx.x], h_key[threadIdx.x], (tag & 1) != 0});
     }
     if (M == LpMode::Refine) {
         __shared__ unsigned long long tsum;
@@ -1147,6 +1143,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                           const BinConfig& bins, hipStream_t s, const uint8_t* include) {
     KH_REQUIRE(S >= 1 && S <= kMaxSub, "sub-rounds must be in [1, 8]");
     KH_REQUIRE(bins.hi[kNumBins - 2] <= 512, "the wave bin holds degrees <= 512");
+    KH_REQUIRE(g.n < (1 << 30) && num_labels < (1 << 30), "labels must be below 2^30");
     subrounds_ = S;
     num_labels_ = num_labels;
     dense_wave_ = mode == LpMode::Refine && num_labels <= kWaveDense;
