@@ -1211,7 +1211,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     }
     hub_total_ = 0;
     dense_total_ = 0;
-    for (int sr = 0; sr < S; ++sr) hub_chunks_[sr] = hub_multi_n_[sr] = mid_n_[sr] = 0;
+    for (int sr = 0; sr < S; ++sr) {
+        hub_chunks_[sr] = hub_multi_n_[sr] = mid_n_[sr] = 0;
+        mid_edges_[sr] = 0;
+    }
     if (hubs == 0) {
         KH_CHECK(hipGetLastError());
         return;
@@ -1294,11 +1297,13 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         int c = 0, t = 0;
         (void)0;
         mid_base_[sr] = mids;
+        mid_edges_[sr] = 0;
         for (int i = 0; i < hub_n_[sr]; ++i) {
             const int d = deg[hub_base_[sr] + i];
             prefix.push_back(c);
             if (d <= kMidChunk) {  // the mid-size kernel, no chunk of the big one
                 midpos[hub_base_[sr] + i] = mids++ - mid_base_[sr];
+                mid_edges_[sr] += d;
                 continue;
             }
             const int nch = (d + kHubChunk - 1) / kHubChunk;
@@ -1520,8 +1525,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 const int* dc = fr ? ccount_.p + l : nullptr;
                 // algorithmic bytes: 12 B per traversed edge (target, weight, label gather) and
                 // 16 B per node (two offsets, weight, own label); label weights are O(moved).
-                const double bytes = cl ? 12.0 * (double)host_edges[l] + 16.0 * cnt : 0.0;
-                ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? host_edges[l] : 0, s);
+                // (the hub list's mid-size hubs are timed in a scope of their own)
+                const bool hub = b == kNumBins - 1;
+                const long long ledges = host_edges[l] - (hub ? mid_edges_[sr] : 0);
+                const int lnodes = cnt - (hub ? mid_n_[sr] : 0);
+                const double bytes = cl ? 12.0 * (double)ledges + 16.0 * lnodes : 0.0;
+                ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? ledges : 0, s);
                 switch (b) {
                     case 0:
                         if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
@@ -1572,12 +1581,14 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                             else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         }
+                        ps.close();  // the hub scope times the chunked kernel alone
                         if (mid_n_[sr] > 0) {
+                            const double mb = cl ? 12.0 * (double)mid_edges_[sr] + 16.0 * mid_n_[sr] : 0.0;
+                            ProfScope pm(prof, base + "_propose_hub_mid", mb, cl ? mid_edges_[sr] : 0, s);
                             const int4* mm = mmap_.p + mid_base_[sr];
                             if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                             else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         }
-                        ps.close();  // the hub scope times propose_hub_chunk alone
                         if (hub_multi_n_[sr] > 0) {
                             ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);
                             const int* ml = multi_.p + multi_base_[sr];

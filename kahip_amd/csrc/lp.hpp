@@ -85,6 +85,7 @@ private:
     int tmap_base_[kMaxSub] = {};   // offset into tmap_ (tasks of the earlier sub-rounds)
     int mid_n_[kMaxSub] = {};       // hubs of the mid-size kernel per sub-round
     int mid_base_[kMaxSub] = {};    // offset into mmap_
+    long long mid_edges_[kMaxSub] = {};  // edges of those hubs
     int multi_base_[kMaxSub] = {};  // offset into multi_
     int task_base_[kMaxSub] = {};   // offset into task_prefix_
     int hub_tasks_[kMaxSub] = {};   // (hub, bucket) tasks per sub-round
