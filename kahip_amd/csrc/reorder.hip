@@ -38,32 +38,59 @@ __global__ void invert_and_degrees(int n, const int* __restrict__ order,
     nvwgt[i] = vwgt[u];
 }
 
-// Each thread fills 8 consecutive entries of the new CSR (row found by binary search).
-__global__ void copy_rows(int n, long long m2, const int* __restrict__ nxadj,
-                          const int* __restrict__ order, const int* __restrict__ xadj,
-                          const int* __restrict__ adjncy, const int* __restrict__ adjwgt,
-                          const int* __restrict__ inv, int* __restrict__ nadj,
-                          int* __restrict__ nw) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long e0 = t * 8;
-    if (e0 >= m2) return;
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = lo + ((hi - lo) >> 1);
-        if ((long long)nxadj[mid] <= e0) lo = mid; else hi = mid;
-    }
-    int i = lo;
-    long long next = nxadj[i + 1];
-    long long src = (long long)xadj[order[i]] - nxadj[i];
-    const long long e1 = min(e0 + 8, m2);
-    for (long long e = e0; e < e1; ++e) {
-        while (e >= next) {
-            ++i;
-            next = nxadj[i + 1];
-            src = (long long)xadj[order[i]] - nxadj[i];
+// Workgroup w fills the new CSR entries [w * kCopyTile, (w + 1) * kCopyTile): the rows that
+// overlap the tile (at most kCopyTile + 1 of them, rows being non-empty inside the range) are
+// staged in LDS as (new offset, old offset), then thread t takes entries t, t + 256, ... and
+// finds its row by a binary search in LDS, so consecutive threads read and write consecutive
+// entries.
+constexpr int kCopyTile = 2048;
+
+__global__ __launch_bounds__(256) void copy_rows(int n, long long m2,
+                                                 const int* __restrict__ nxadj,
+                                                 const int* __restrict__ order,
+                                                 const int* __restrict__ xadj,
+                                                 const int* __restrict__ adjncy,
+                                                 const int* __restrict__ adjwgt,
+                                                 const int* __restrict__ inv,
+                                                 int* __restrict__ nadj, int* __restrict__ nw) {
+    __shared__ int nbeg[kCopyTile + 1];  // new row offsets of the tile's rows
+    __shared__ int obeg[kCopyTile + 1];  // their old row offsets
+    __shared__ int r0s, nrow;
+    const long long t0 = (long long)blockIdx.x * kCopyTile;
+    const long long t1 = min(t0 + kCopyTile, m2);
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = n - 1;  // last row with nxadj[row] <= t0
+        while (lo < hi) {
+            const int mid = lo + ((hi - lo + 1) >> 1);
+            if ((long long)nxadj[mid] <= t0) lo = mid; else hi = mid - 1;
         }
-        nadj[e] = inv[adjncy[src + e]];
-        nw[e] = adjwgt[src + e];
+        int hi2 = n - 1, lo2 = lo;  // last row with nxadj[row] < t1
+        while (lo2 < hi2) {
+            const int mid = lo2 + ((hi2 - lo2 + 1) >> 1);
+            if ((long long)nxadj[mid] < t1) lo2 = mid; else hi2 = mid - 1;
+        }
+        r0s = lo;
+        // rows are sorted by degree, descending, so empty rows come last and never start
+        // inside a tile: every row but the first starts at its own offset in (t0, t1), hence
+        // at most kCopyTile rows (the clamp only guards the LDS arrays)
+        nrow = min(lo2 - lo + 1, kCopyTile + 1);
+    }
+    __syncthreads();
+    const int r0 = r0s, nr = nrow;
+    for (int i = threadIdx.x; i < nr; i += 256) {
+        nbeg[i] = nxadj[r0 + i];
+        obeg[i] = xadj[order[r0 + i]];
+    }
+    __syncthreads();
+    for (long long e = t0 + threadIdx.x; e < t1; e += 256) {
+        int lo = 0, hi = nr - 1;  // last staged row starting at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((long long)nbeg[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        const long long src = (long long)obeg[lo] + (e - nbeg[lo]);
+        nadj[e] = inv[adjncy[src]];
+        nw[e] = adjwgt[src];
     }
 }
 
@@ -115,9 +142,9 @@ void reorder_by_degree(const DGraph& g, OwnedGraph& out, DevBuf<int>& inv, hipSt
                                      rocprim::plus<int>(), s));
     out.alloc_edges(g.m2);
     if (g.m2 > 0)
-        hipLaunchKernelGGL(copy_rows, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256), 0, s, n,
-                           g.m2, out.g.xadj, order.p, g.xadj, g.adjncy, g.adjwgt, inv.p,
-                           out.g.adjncy, out.g.adjwgt);
+        hipLaunchKernelGGL(copy_rows, dim3(grid_for(g.m2, kCopyTile)), dim3(256), 0, s, n, g.m2,
+                           out.g.xadj, order.p, g.xadj, g.adjncy, g.adjwgt, inv.p, out.g.adjncy,
+                           out.g.adjwgt);
     out.g.total_vwgt = g.total_vwgt;
     out.g.max_degree = g.max_degree;
     KH_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
