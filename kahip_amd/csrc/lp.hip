@@ -94,6 +94,28 @@ __device__ inline void reduce_cand(Cand& c) {
     }
 }
 
+// Rotation within each row of 16 lanes (DPP row_ror: a full-rate VALU move, no LDS crossbar).
+template <int R>
+__device__ inline int row_ror(int x) {
+    if constexpr (R == 0) return x;
+    else return __builtin_amdgcn_update_dpp(0, x, 0x120 + R, 0xF, 0xF, false);
+}
+
+// Adds, for rotations R..15 of (l0, w0, i0) within the lane's row, the weight w0 of every lane
+// whose label l0 equals lab; a lane of lower id i0 with the same label clears leader. The ids are
+// rotated with the values, so the result does not depend on the rotation's direction.
+template <int R>
+__device__ inline void agg_rot(int lab, int gl, int l0, int w0, int i0, int& conn, bool& leader) {
+    if constexpr (R < 16) {
+        const int lr = row_ror<R>(l0), wr = row_ror<R>(w0), ir = row_ror<R>(i0);
+        if (lr == lab) {
+            conn += wr;
+            if (ir < gl) leader = false;
+        }
+        agg_rot<R + 1>(lab, gl, l0, w0, i0, conn, leader);
+    }
+}
+
 template <int W>
 __device__ inline int reduce_max(int x) {
 #pragma unroll
@@ -179,15 +201,27 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
         lab = a.label[a.adjncy[e]];
         w = a.adjwgt[e];
     }
+    // conn = summed weight of the group's lanes with this lane's label; leader = the lowest of
+    // them. Groups of 16, 32 and 64 lanes are whole DPP rows: rotations within the row, then
+    // within each other row of the group after one swap (xor 16, 32, 48), visit every lane of
+    // the group once.
     int conn = 0;
     bool leader = lab >= 0;
+    if constexpr (G >= 16) {
+        conn = w;
+        agg_rot<1>(lab, gl, lab, w, gl, conn, leader);
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        const int lj = __shfl(lab, j, G);
-        const int wj = __shfl(w, j, G);
-        if (lj == lab) {
-            conn += wj;
-            if (j < gl) leader = false;
+        for (int x = 16; x < G; x += 16)
+            agg_rot<0>(lab, gl, __shfl_xor(lab, x, G), __shfl_xor(w, x, G), gl ^ x, conn, leader);
+    } else {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int lj = __shfl(lab, j, G);
+            const int wj = __shfl(w, j, G);
+            if (lj == lab) {
+                conn += wj;
+                if (j < gl) leader = false;
+            }
         }
     }
     const int conn_own = reduce_max<G>((lab >= 0 && lab == own) ? conn : 0);
