@@ -521,7 +521,8 @@ __global__ __launch_bounds__(256) void hub_dense_decide(
 template <LpMode M, int THREADS, int CHUNK>
 __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
     LpArgs a, const int4* __restrict__ cmap, int2* __restrict__ partials,
-    const long long* __restrict__ offm_base, int* __restrict__ offm) {
+    const long long* __restrict__ offm_base, int* __restrict__ offm,
+    const int* __restrict__ ct_base, int* __restrict__ cntT) {
     constexpr int CAP = 2 * CHUNK;
     constexpr bool MULTI = CHUNK == kHubChunk;  // hubs of several chunks come here
     static_assert(!MULTI || THREADS == kHubThreads, "the multi-chunk path assumes kHubThreads");
@@ -609,11 +610,16 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
         }
         __syncthreads();
         int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
-        for (int b = threadIdx.x; b < P; b += kHubThreads) offs[b] = bcur[b];
+        int* cnt_t = cntT + ct_base[gi] + c_idx;  // bucket-major: [b * nchunks + c_idx]
+        for (int b = threadIdx.x; b < P; b += kHubThreads) {
+            offs[b] = bcur[b];
+            if (b + 1 < P) cnt_t[(long long)b * nchunks] = bcur[b + 1] - bcur[b];
+        }
         if (threadIdx.x == 0) {
             int tot = 0;
             for (int w = 0; w < kHubThreads / 64; ++w) tot += wsum[w];
             offs[P] = tot;
+            cnt_t[(long long)(P - 1) * nchunks] = tot - bcur[P - 1];
         }
         __syncthreads();
         int2* out = partials + (long long)chunk * kHubChunk;
@@ -648,42 +654,43 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
     }
 }
 
-// Exclusive scan of one int per thread over the workgroup; returns the thread's prefix and
-// leaves the sum in *total (wsum: THREADS / 64 ints of LDS).
-template <int THREADS>
-__device__ inline int block_excl_scan(int x, int* wsum, int* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int incl = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
+// Moves the bucket-grouped entries of every chunk of a multi-chunk hub to their bucket-major
+// position (posT = exclusive scan of the bucket-major counts), so that every bucket of a hub is
+// one contiguous range.
+__global__ __launch_bounds__(256) void hub_scatter(
+    const int4* __restrict__ cmap, const long long* __restrict__ offm_base,
+    const int* __restrict__ offm, const int* __restrict__ ct_base, const int* __restrict__ posT,
+    const int2* __restrict__ partials, int2* __restrict__ bucketed) {
+    const int chunk = blockIdx.x;
+    const int4 cm = cmap[chunk];
+    const int gi = cm.x;
+    const int nch = cm.z;
+    if (nch <= 1) return;
+    const int c_idx = cm.y;
+    const int P = min(kBucketsPerChunk * nch, kMaxBuckets);
+    const int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
+    const int* pos = posT + ct_base[gi] + c_idx;
+    const int2* src = partials + (long long)chunk * kHubChunk;
+    const int total = offs[P];
+    for (int i = threadIdx.x; i < total; i += 256) {
+        const int2 kv = src[i];
+        const int b = bucket_of(kv.x, P);
+        bucketed[pos[(long long)b * nch] + (i - offs[b])] = kv;
     }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    int before = 0, all = 0;
-    for (int w = 0; w < THREADS / 64; ++w) {
-        if (w < wave) before += wsum[w];
-        all += wsum[w];
-    }
-    __syncthreads();  // wsum is free again when this returns
-    *total = all;
-    return before + incl - x;
 }
 
-// Merge of one (multi-chunk hub, bucket) task: bucket b's entries of every chunk of the hub
-// (chunk c's run [offm_c[b], offm_c[b + 1]) of its partials) are summed in an LDS hash (or, past
-// kFinLds entries, in the hub's global table, filtering by bucket); the bucket's best candidate
-// and, for the bucket of the own label, conn(own) are left in bres / bown. The chunks' runs are
-// read THREADS chunks at a time: their lengths are scanned in LDS and each thread takes every
-// THREADS-th entry of the group. SMALL: kSmallThreads threads and 32 KB of LDS for buckets of
-// <= kSmallMax entries; larger buckets are queued for the large variant (1024 threads, 128 KB
-// of LDS), which grid-strides over that queue.
+// Merge of one (multi-chunk hub, bucket) task: the bucket's entries of all chunks of the hub are
+// summed in an LDS hash (or, past kFinLds entries, in the hub's global table, filtering by
+// bucket); the bucket's best candidate and, for the bucket of the own label, conn(own) are left
+// in bres / bown. SMALL: 256 threads and 16 KB of LDS for buckets of <= kSmallMax entries;
+// larger buckets are queued for the large variant (1024 threads, 128 KB of LDS), which
+// grid-strides over that queue.
 template <LpMode M, bool SMALL>
 __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
-    LpArgs a, int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
-    const long long* __restrict__ offm_base, const int* __restrict__ offm,
-    const int2* __restrict__ partials, const long long* __restrict__ tab_off,
+    LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
+    int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
+    const int* __restrict__ chunk_prefix, const int* __restrict__ ct_base, const int* __restrict__ posT,
+    const int2* __restrict__ bucketed, const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
     int* __restrict__ bown, uint8_t* __restrict__ fb, int* __restrict__ big_tasks,
     int* __restrict__ n_big_tasks) {
@@ -691,32 +698,23 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
     constexpr int CAP = SMALL ? kSmallCap : kFinCap;
     __shared__ int K[CAP];
     __shared__ int V[CAP];
-    __shared__ int cpre[THREADS];  // exclusive prefix of the group's run lengths
-    __shared__ int clo[THREADS];   // start of each chunk's run in its partials
-    __shared__ int wsum[THREADS / 64];
     const int n_iter = SMALL ? n_tasks : *n_big_tasks;
     for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
         const int task = SMALL ? it : big_tasks[it];
-        const int4 tm = tmap[task];  // {hub index, node, first chunk of the hub, chunks}
+        const int4 tm = tmap[task];  // {first count index, one past the last, hub index, node}
         const int2 ts = tsub[task];  // {multi-chunk hub index, bucket | buckets << 16}
         const int mi = ts.x;
-        const int gi = tm.x;
-        const int v = tm.y;
+        const int gi = tm.z;
+        const int v = tm.w;
         if (M == LpMode::Refine && a.active[v] == 0) {  // uniform per block
             __syncthreads();
             continue;
         }
-        const int chunk0 = tm.z, nch = tm.w;
         const int P = ts.y >> 16;
         const int b = ts.y & 0xffff;
-        const int* offs = offm + offm_base[gi];  // chunk c: offs[c * (P + 1) .. + P]
-        int mine = 0;
-        for (int c = threadIdx.x; c < nch; c += THREADS) {
-            const int* o = offs + (long long)c * (P + 1) + b;
-            mine += o[1] - o[0];
-        }
-        int total = 0;
-        (void)block_excl_scan<THREADS>(mine, wsum, &total);
+        const int r0 = posT[tm.x];
+        const int r1 = posT[tm.y];
+        const int total = r1 - r0;
         if (SMALL && total > kSmallMax) {
             if (threadIdx.x == 0) big_tasks[atomicAdd(n_big_tasks, 1)] = task;
             __syncthreads();
@@ -742,39 +740,17 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
             if (threadIdx.x == 0) fb[mi] = 1;
         }
         __syncthreads();
-        for (int g0 = 0; g0 < nch; g0 += THREADS) {
-            const int c = g0 + threadIdx.x;
-            int lo = 0, len = 0;
-            if (c < nch) {
-                const int* o = offs + (long long)c * (P + 1) + b;
-                lo = o[0];
-                len = o[1] - lo;
-            }
-            int gtotal = 0;
-            const int pre = block_excl_scan<THREADS>(len, wsum, &gtotal);
-            cpre[threadIdx.x] = pre;
-            clo[threadIdx.x] = lo;
-            __syncthreads();
-            const int ng = min(THREADS, nch - g0);
-            for (int t = threadIdx.x; t < gtotal; t += THREADS) {
-                int l = 0, h = ng - 1;  // last chunk of the group whose run starts at or before t
-                while (l < h) {
-                    const int mid = (l + h + 1) >> 1;
-                    if (cpre[mid] <= t) l = mid; else h = mid - 1;
+        for (int i = r0 + threadIdx.x; i < r1; i += THREADS) {
+            const int2 kv = bucketed[i];
+            long long h = (long long)(mix32((uint32_t)kv.x) & (uint32_t)(cap - 1));
+            while (true) {
+                const int old = atomicCAS(&TK[h], -1, kv.x);
+                if (old == -1 || old == kv.x) {
+                    atomicAdd(&TV[h], kv.y);
+                    break;
                 }
-                const int2 kv = partials[(long long)(chunk0 + g0 + l) * kHubChunk + clo[l] +
-                                         (t - cpre[l])];
-                long long hh = (long long)(mix32((uint32_t)kv.x) & (uint32_t)(cap - 1));
-                while (true) {
-                    const int old = atomicCAS(&TK[hh], -1, kv.x);
-                    if (old == -1 || old == kv.x) {
-                        atomicAdd(&TV[hh], kv.y);
-                        break;
-                    }
-                    hh = (hh + 1) & (cap - 1);
-                }
+                h = (h + 1) & (cap - 1);
             }
-            __syncthreads();  // cpre / clo are rewritten by the next group
         }
         // Only the global table needs the device-scope fence (it waits for the atomics and
         // invalidates the vector L1); it is costly, so the LDS path, the common one, skips it.
@@ -1104,21 +1080,24 @@ __global__ void build_mid_map(const int* __restrict__ list, int count,
 }
 
 // (hub, bucket) task t of a sub-round -> {multi-chunk hub index, bucket, buckets, hub index}.
-// tmap[t] = {hub index, node, first chunk of the hub in the sub-round's launch, chunks of the
-// hub}; tsub[t] = {multi-chunk hub index, bucket | buckets << 16}.
+// tmap[t] = {index of the bucket's first count in the bucket-major table, index one past its
+// last, hub index, node}; tsub[t] = {multi-chunk hub index, bucket | buckets << 16}.
 __global__ void build_task_map(const int* __restrict__ list, const int* __restrict__ multi,
                                int n_multi, const int* __restrict__ task_prefix,
-                               const int* __restrict__ chunk_prefix, int4* __restrict__ tmap,
+                               const int* __restrict__ chunk_prefix,
+                               const int* __restrict__ ct_base, int4* __restrict__ tmap,
                                int2* __restrict__ tsub) {
     const int mi = blockIdx.x * blockDim.x + threadIdx.x;
     if (mi >= n_multi) return;
     const int t0 = task_prefix[mi], t1 = task_prefix[mi + 1];
     const int gi = multi[mi];
-    const int c0 = chunk_prefix[gi], nch = chunk_prefix[gi + 1] - c0;
+    const int nch = chunk_prefix[gi + 1] - chunk_prefix[gi];
+    const int cb = ct_base[gi];
     const int v = list[gi];
     for (int t = t0; t < t1; ++t) {
-        tmap[t] = make_int4(gi, v, c0, nch);
-        tsub[t] = make_int2(mi, (t - t0) | ((t1 - t0) << 16));
+        const int b = t - t0;
+        tmap[t] = make_int4(cb + b * nch, cb + (b + 1) * nch, gi, v);
+        tsub[t] = make_int2(mi, b | ((t1 - t0) << 16));
     }
 }
 
@@ -1341,14 +1320,16 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     }
     std::vector<int> prefix, multi, tasks, midpos(hubs, -1);
     std::vector<long long> toff(hubs, 0), tcap(hubs, 0), obase(hubs, 0);
+    std::vector<int> ctb(hubs, 0);
     int mids = 0;
-    long long offm_total = 0;
+    long long offm_total = 0, ct_total = 0, ct_sr_start = 0;
     int max_tasks = 0, max_multi = 0;
     for (int sr = 0; sr < S; ++sr) {
         chunk_base_[sr] = (int)prefix.size();
         multi_base_[sr] = (int)multi.size();
         task_base_[sr] = (int)tasks.size();
         int c = 0, t = 0;
+        (void)0;
         mid_base_[sr] = mids;
         mid_edges_[sr] = 0;
         for (int i = 0; i < hub_n_[sr]; ++i) {
@@ -1370,6 +1351,8 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                 hub_total_ += cap;
                 obase[hub_base_[sr] + i] = offm_total;
                 offm_total += (long long)nch * (P + 1);
+                ctb[hub_base_[sr] + i] = (int)ct_total;
+                ct_total += (long long)nch * P;
                 multi.push_back(i);
                 tasks.push_back(t);
                 t += P;
@@ -1377,6 +1360,10 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         }
         prefix.push_back(c);
         tasks.push_back(t);
+        ct_sr_base_[sr] = (int)ct_sr_start;
+        ct_sr_size_[sr] = (int)(ct_total - ct_sr_start);
+        ct_total += 1;  // zero sentinel after the sub-round's counts
+        ct_sr_start = ct_total;
         hub_chunks_[sr] = c;
         mid_n_[sr] = mids - mid_base_[sr];
         hub_multi_n_[sr] = (int)multi.size() - multi_base_[sr];
@@ -1391,6 +1378,23 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     task_prefix_.reserve((long long)tasks.size());
     multi_.reserve((long long)multi.size());
     offm_.reserve(offm_total);
+    KH_REQUIRE(ct_total < (1LL << 31), "hub bucket table exceeds 2^31 entries");
+    ct_base_.reserve(hubs);
+    cnt_t_.reserve(ct_total);
+    pos_t_.reserve(ct_total);
+    fill_async(cnt_t_.p, ct_total, 0, s);  // sentinels stay zero
+    KH_CHECK(hipMemcpyAsync(ct_base_.p, ctb.data(), sizeof(int) * hubs, hipMemcpyHostToDevice,
+                            s));
+    {
+        size_t bytes = 0, need = 0;
+        for (int sr = 0; sr < S; ++sr) {
+            KH_CHECK(rocprim::exclusive_scan(nullptr, bytes, cnt_t_.p, pos_t_.p, 0,
+                                             (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
+            need = std::max(need, bytes);
+        }
+        scan_tmp_.reserve((long long)need + 1);
+        scan_tmp_bytes_ = need;
+    }
     bres_.reserve(max_tasks);
     bown_.reserve(max_tasks);
     big_tasks_.reserve(max_tasks);
@@ -1434,8 +1438,8 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                                    0, s, lists_.p + host_offsets[sr * kNumBins + kNumBins - 1],
                                    multi_.p + multi_base_[sr], hub_multi_n_[sr],
                                    task_prefix_.p + task_base_[sr],
-                                   chunk_prefix_.p + chunk_base_[sr], tmap_.p + tb,
-                                   tsub_.p + tb);
+                                   chunk_prefix_.p + chunk_base_[sr], ct_base_.p + hub_base_[sr],
+                                   tmap_.p + tb, tsub_.p + tb);
             tmap_base_[sr] = (int)tb;
             tb += hub_tasks_[sr];
         }
@@ -1449,6 +1453,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     int max_chunks = 0;
     for (int sr = 0; sr < S; ++sr) max_chunks = std::max(max_chunks, hub_chunks_[sr]);
     partials_.reserve((long long)max_chunks * kHubChunk);
+    bucketed_.reserve((long long)max_chunks * kHubChunk);
     KH_CHECK(hipStreamSynchronize(s));  // the host vectors above must outlive the copies
     KH_CHECK(hipGetLastError());
 }
@@ -1601,20 +1606,22 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             }
                             break;
                         }
+                        const int* cp = chunk_prefix_.p + chunk_base_[sr];
                         const int4* cm = cmap_.p + cmap_base_[sr];
                         const long long* to = tab_off_.p + hub_base_[sr];
                         const long long* ob = offm_base_.p + hub_base_[sr];
+                        const int* ctb = ct_base_.p + hub_base_[sr];
                         if (hub_chunks_[sr] > 0) {
-                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p);
-                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p);
+                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kHubThreads, kHubChunk>), dim3(hub_chunks_[sr]), dim3(kHubThreads), 0, s, a, cm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         }
                         ps.close();  // the hub scope times the chunked kernel alone
                         if (mid_n_[sr] > 0) {
                             const double mb = cl ? 12.0 * (double)mid_edges_[sr] + 16.0 * mid_n_[sr] : 0.0;
                             ProfScope pm(prof, base + "_propose_hub_mid", mb, cl ? mid_edges_[sr] : 0, s);
                             const int4* mm = mmap_.p + mid_base_[sr];
-                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p);
-                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p);
+                            if (cl) hipLaunchKernelGGL((propose_hub_chunk<LpMode::Cluster, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
+                            else hipLaunchKernelGGL((propose_hub_chunk<LpMode::Refine, kMidThreads, kMidChunk>), dim3(mid_n_[sr]), dim3(kMidThreads), 0, s, a, mm, partials_.p, ob, offm_.p, ctb, cnt_t_.p);
                         }
                         if (hub_multi_n_[sr] > 0) {
                             ProfScope pf(prof, base + "_propose_hub_merge", 0.0, hub_tasks_[sr], s);
@@ -1625,15 +1632,20 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             const int nm = hub_multi_n_[sr];
                             Cand* br = reinterpret_cast<Cand*>(bres_.p);
                             const int nt = hub_tasks_[sr];
+                            // bucket-major positions, relative to this sub-round's block of counts
+                            size_t sb = scan_tmp_bytes_;
+                            KH_CHECK(rocprim::exclusive_scan(scan_tmp_.p, sb, cnt_t_.p + ct_sr_base_[sr], pos_t_.p + ct_sr_base_[sr], 0, (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
+                            const int* pt = pos_t_.p;
+                            hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, cm, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

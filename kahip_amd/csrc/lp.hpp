@@ -89,9 +89,14 @@ private:
     int multi_base_[kMaxSub] = {};  // offset into multi_
     int task_base_[kMaxSub] = {};   // offset into task_prefix_
     int hub_tasks_[kMaxSub] = {};   // (hub, bucket) tasks per sub-round
+    int ct_sr_base_[kMaxSub] = {};  // bucket-major count block of each sub-round
+    int ct_sr_size_[kMaxSub] = {};
+    size_t scan_tmp_bytes_ = 0;
     DevBuf<int> hub_deg_, chunk_prefix_, multi_, big_, n_big_;
     DevBuf<long long> tab_off_, tab_cap_;
-    DevBuf<int2> partials_;  // chunk partials, grouped by bucket within each chunk
+    DevBuf<int2> partials_, bucketed_;
+    DevBuf<int> ct_base_, cnt_t_, pos_t_;
+    DevBuf<char> scan_tmp_;
     DevBuf<long long> offm_base_;
     DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_;
     DevBuf<int4> bres_;
