@@ -147,6 +147,64 @@ def test_contract_chunked_tier(case):
         assert np.array_equal(g_, w_)
 
 
+def sparse_random_graph(n, m, seed):
+    """Random simple graph with about m undirected edges, weighted, CSR with sorted rows."""
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, 2 * m)
+    v = rng.integers(0, n, 2 * m)
+    keep = u != v
+    u, v = np.minimum(u[keep], v[keep]), np.maximum(u[keep], v[keep])
+    key = np.unique(u.astype(np.int64) * n + v)[:m]
+    u, v = (key // n).astype(np.int64), (key % n).astype(np.int64)
+    w = rng.integers(1, 9, len(u))
+    uu, vv, ww = np.concatenate([u, v]), np.concatenate([v, u]), np.concatenate([w, w])
+    order = np.lexsort((vv, uu))
+    xadj = np.zeros(n + 1, np.int64)
+    np.add.at(xadj, uu[order] + 1, 1)
+    return (np.cumsum(xadj).astype(np.int32), vv[order].astype(np.int32),
+            rng.integers(1, 4, n).astype(np.int32), ww[order].astype(np.int32))
+
+
+def test_contract_lds_tiers():
+    """Coarse rows of many low-degree members whose distinct neighbours exceed the wave tier:
+    <= 1024 distinct (256-thread tier), <= 4096 (1024-thread tier, several member batches) and
+    beyond (global tier), all in one clustering."""
+    x, a, vw, ew = sparse_random_graph(40000, 160000, 21)  # average degree 8
+    n = len(x) - 1
+    rng = np.random.default_rng(22)
+    cl = np.arange(n, dtype=np.int32)
+    perm = rng.permutation(n)
+    pos = 0
+    for size in (40, 90, 120, 250, 400, 520, 700, 2000):  # summed degrees ~8 x size
+        cl[perm[pos:pos + size]] = perm[pos]
+        pos += size
+    cl, nc = O.remap_first_appearance(cl)
+    got = K.contract(x, a, vw, ew, cl, nc)
+    want = O.contract(x, a, vw, ew, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+    # one row of 1500 members of degree 2 into 600 targets: ~600 distinct, 3000 member edges,
+    # more members than the 256-thread tier stages at once (two batches)
+    rng = np.random.default_rng(23)
+    t = 1500 + rng.integers(0, 600, (1500, 2))
+    t[:, 1] = np.where(t[:, 1] == t[:, 0], 1500 + (t[:, 0] - 1500 + 1) % 600, t[:, 1])
+    u = np.repeat(np.arange(1500), 2)
+    v = t.reshape(-1)
+    uu, vv = np.concatenate([u, v]), np.concatenate([v, u])
+    order = np.lexsort((vv, uu))
+    xs = np.zeros(2101, np.int64)
+    np.add.at(xs, uu[order] + 1, 1)
+    xs = np.cumsum(xs).astype(np.int32)
+    as_ = vv[order].astype(np.int32)
+    cl = np.arange(2100, dtype=np.int32)
+    cl[:1500] = 0
+    cl, nc = O.remap_first_appearance(cl)
+    got = K.contract(xs, as_, None, None, cl, nc)
+    want = O.contract(xs, as_, None, None, cl, nc)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+
+
 REFINE_CASES = [("er200w", 2, 0), ("rgg15", 2, 1), ("rgg15", 64, 2), ("delaunay15", 8, 3),
                 ("weighted132", 4, 4), ("star1500", 4, 5), ("rmat12", 16, 6), ("star10k", 4, 7),
                 ("star10k", 5000, 8), ("rmat12", 3000, 9), ("rmat15", 64, 10), ("rmat15", 8, 11)]
