@@ -34,7 +34,6 @@ namespace kahip {
 namespace {
 
 constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
-constexpr int kRowMaxDistinct = 512;
 constexpr int kRowSmall = 128;    // rows with summed member degree <= this use 256 slots
 constexpr int kDirectGlobal = 4096;  // rows with more member edges go to the chunked tier
 constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
@@ -261,13 +260,13 @@ __global__ __launch_bounds__(64) void rows_lds(
     if (lane == 0) cdeg[p] = cnt;
 }
 
-// Rows with more than kRowMaxDistinct distinct neighbours: one workgroup of THREADS per row
-// (grid-stride over the list), an LDS hash of up to CAP slots and an LDS bitonic sort of up to
-// MAXD entries; rows with more distinct neighbours are passed on to the next tier. The members'
-// edges are expanded with a load-balanced scan, MAXD members at a time: S holds each member's
-// (inclusive degree prefix, row start), every thread takes every THREADS-th edge of the batch
-// and gathers kEpl targets at a time, so the target and cluster loads of many edges are in
-// flight together. After the hash phase S holds the sort keys.
+// Rows with more distinct neighbours than the wave tier holds (kRowCap / 2): one workgroup of
+// THREADS per row (grid-stride over the list), an LDS hash of up to CAP slots and an LDS bitonic
+// sort of up to MAXD entries; rows with more distinct neighbours are passed on to the next tier.
+// The members' edges are expanded with a load-balanced scan, MAXD members at a time: S holds
+// each member's (inclusive degree prefix, row start), every thread takes every THREADS-th edge
+// of the batch and gathers kEpl targets at a time, so the target and cluster loads of many
+// edges are in flight together. After the hash phase S holds the sort keys.
 template <int THREADS, int CAP, int MAXD>
 __global__ __launch_bounds__(THREADS) void rows_lds_big(
     const int* __restrict__ list, const int* __restrict__ list_count, const int* __restrict__ xadj,
@@ -555,12 +554,12 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(incl, o, 64);
-        if ((threadIdx.x & 63) >= o) incl += y;
+        if ((int)(threadIdx.x & 63) >= o) incl += y;
     }
     if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
     __syncthreads();
     int wbase = 0;
-    for (int w = 0; w < (threadIdx.x >> 6); ++w) wbase += wsum[w];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wbase += wsum[w];
     int acc = wbase + incl - run;
     for (int b = b0; b < b1; ++b) {
         const int cnt = bcur[b];

@@ -596,12 +596,12 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(incl, o, 64);
-            if ((threadIdx.x & 63) >= o) incl += y;
+            if ((int)(threadIdx.x & 63) >= o) incl += y;
         }
         if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
         __syncthreads();
         int wbase = 0;
-        for (int w = 0; w < (threadIdx.x >> 6); ++w) wbase += wsum[w];
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wbase += wsum[w];
         int acc = wbase + incl - run;
         for (int b = b0; b < b1; ++b) {
             const int cnt = bcur[b];
@@ -687,10 +687,9 @@ __global__ __launch_bounds__(256) void hub_scatter(
 // grid-strides over that queue.
 template <LpMode M, bool SMALL>
 __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
-    LpArgs a, const int* __restrict__ list, const int* __restrict__ multi, int n_multi,
-    int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
-    const int* __restrict__ chunk_prefix, const int* __restrict__ ct_base, const int* __restrict__ posT,
-    const int2* __restrict__ bucketed, const long long* __restrict__ tab_off,
+    LpArgs a, int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
+    const int* __restrict__ posT, const int2* __restrict__ bucketed,
+    const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
     int* __restrict__ bown, uint8_t* __restrict__ fb, int* __restrict__ big_tasks,
     int* __restrict__ n_big_tasks) {
@@ -1527,7 +1526,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         const size_t lds = sizeof(int) * (size_t)copies * num_labels_;
         for (int sr = 0; sr < S; ++sr) {
             if (hub_multi_n_[sr] == 0) continue;
-            const int l = sr * kNumBins + kNumBins - 1;
             hipLaunchKernelGGL(propose_hub_dense_chunk, dim3(hub_chunks_[sr]), dim3(kHubDenseThreads), lds, s, a, cmap_.p + cmap_base_[sr], num_labels_, dense_off_.p + hub_base_[sr], dense_cnt_.p, 1, 0);
         }
     }
@@ -1606,7 +1604,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             }
                             break;
                         }
-                        const int* cp = chunk_prefix_.p + chunk_base_[sr];
                         const int4* cm = cmap_.p + cmap_base_[sr];
                         const long long* to = tab_off_.p + hub_base_[sr];
                         const long long* ob = offm_base_.p + hub_base_[sr];
@@ -1640,12 +1637,12 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, lst, ml, nm, nt, tm, tsb, cp, ctb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

@@ -138,12 +138,6 @@ __global__ void expansion_star2_kernel(int n, long long m2, const int* __restric
     }
 }
 
-__global__ void quantize_kernel(long long m, const float* __restrict__ logit, int shift,
-                                int* __restrict__ score) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) score[i] = max(quantize_logit(logit[i]) >> shift, 1);
-}
-
 __global__ void zero_int_kernel(int* p) { *p = 0; }
 
 __global__ void max_degree_kernel(int n, const int* __restrict__ xadj, int* __restrict__ out) {
