@@ -37,6 +37,7 @@ constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k 
 constexpr int kWaveSmall = 128;
 constexpr int kBlock = 256;
 constexpr int kFrontierGrid = 2048;  // workgroups of a refinement list kernel (grid-stride)
+constexpr int kGroupItems = 2;       // nodes per lane group in flight in propose_group
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
@@ -179,103 +180,138 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
 // ---------------------------------------------------------------- G lanes per node (deg <= G)
 // dcount: the list's length on the device (the compacted frontier of a refinement round);
 // workgroups then stride over the list's virtual blocks, so a small frontier costs few of them.
+// A virtual block holds kGroupItems x (kBlock / G) nodes: every lane group serves kGroupItems
+// nodes at once, whose dependent loads (list, offsets, targets, labels, label weights) are
+// issued together, so each wave has that many chains in flight.
 template <LpMode M, int G>
 __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
                                                         int count, const int* __restrict__ dcount) {
+    constexpr int IT = kGroupItems;
+    constexpr int NB = kBlock / G;  // nodes per item and workgroup
     if (dcount) count = *dcount;
-    const int nvb = (int)(((long long)count * G + kBlock - 1) / kBlock);
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-    const int tid = vb * kBlock + threadIdx.x;
-    const int gi = tid / G;
+    const int nvb = (int)(((long long)count + IT * NB - 1) / (IT * NB));
     const int gl = threadIdx.x & (G - 1);
-    bool valid = gi < count;
-    const int v = valid ? list[gi] : 0;
-    if (M == LpMode::Refine && valid) valid = a.active[v] != 0;
-    // every lane of the group takes part in the shuffles below: no early exit
-    const int own = valid ? a.label[v] : -1;
-    const int beg = valid ? a.xadj[v] : 0;
-    const int deg = valid ? a.xadj[v + 1] - beg : 0;
-    int lab = -1, w = 0;
-    if (gl < deg) {
-        const int e = beg + gl;
-        lab = a.label[a.adjncy[e]];
-        w = a.adjwgt[e];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    int v[IT], own[IT], beg[IT], deg[IT], lab[IT], w[IT];
+    bool valid[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const long long gi = ((long long)vb * IT + i) * NB + threadIdx.x / G;
+        valid[i] = gi < count;
+        v[i] = valid[i] ? list[gi] : 0;
     }
+    if (M == LpMode::Refine) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) valid[i] = valid[i] && a.active[v[i]] != 0;
+    }
+    // every lane of the group takes part in the shuffles below: no early exit
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        own[i] = valid[i] ? a.label[v[i]] : -1;
+        beg[i] = valid[i] ? a.xadj[v[i]] : 0;
+        deg[i] = valid[i] ? a.xadj[v[i] + 1] - beg[i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        lab[i] = gl < deg[i] ? a.adjncy[beg[i] + gl] : -1;
+        w[i] = gl < deg[i] ? a.adjwgt[beg[i] + gl] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+        if (lab[i] >= 0) lab[i] = a.label[lab[i]];
     // conn = summed weight of the group's lanes with this lane's label; leader = the lowest of
     // them. Groups of 16, 32 and 64 lanes are whole DPP rows: rotations within the row, then
     // within each other row of the group after one swap (xor 16, 32, 48), visit every lane of
     // the group once.
-    int conn = 0;
-    bool leader = lab >= 0;
-    if constexpr (G >= 16) {
-        conn = w;
-        agg_rot<1>(lab, gl, lab, w, gl, conn, leader);
+    int conn[IT];
+    bool leader[IT];
 #pragma unroll
-        for (int x = 16; x < G; x += 16)
-            agg_rot<0>(lab, gl, __shfl_xor(lab, x, G), __shfl_xor(w, x, G), gl ^ x, conn, leader);
-    } else {
+    for (int i = 0; i < IT; ++i) {
+        conn[i] = 0;
+        leader[i] = lab[i] >= 0;
+        if constexpr (G >= 16) {
+            conn[i] = w[i];
+            agg_rot<1>(lab[i], gl, lab[i], w[i], gl, conn[i], leader[i]);
 #pragma unroll
-        for (int j = 0; j < G; ++j) {
-            const int lj = __shfl(lab, j, G);
-            const int wj = __shfl(w, j, G);
-            if (lj == lab) {
-                conn += wj;
-                if (j < gl) leader = false;
+            for (int x = 16; x < G; x += 16)
+                agg_rot<0>(lab[i], gl, __shfl_xor(lab[i], x, G), __shfl_xor(w[i], x, G), gl ^ x,
+                           conn[i], leader[i]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int lj = __shfl(lab[i], j, G);
+                const int wj = __shfl(w[i], j, G);
+                if (lj == lab[i]) {
+                    conn[i] += wj;
+                    if (j < gl) leader[i] = false;
+                }
             }
         }
     }
-    const int conn_own = reduce_max<G>((lab >= 0 && lab == own) ? conn : 0);
-    const int nw = valid ? a.vwgt[v] : 0;
-    Cand c{-1, 0u, 0x7fffffff, 0};
-    if (leader && lab != own) {
-        const int sz = a.size[lab];
-        if ((long long)sz + nw < (long long)bound_of(a, lab)) {
-            c.conn = conn;
-            c.tie = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-            c.lab = lab;
-            c.size = sz;
-        }
+    int conn_own[IT], nw[IT], sz[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        conn_own[i] = reduce_max<G>((lab[i] >= 0 && lab[i] == own[i]) ? conn[i] : 0);
+        nw[i] = valid[i] ? a.vwgt[v[i]] : 0;
+        sz[i] = (leader[i] && lab[i] != own[i]) ? a.size[lab[i]] : 0;
     }
-    reduce_cand<G>(c);
+    Contrib r[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        Cand c{-1, 0u, 0x7fffffff, 0};
+        if (leader[i] && lab[i] != own[i] &&
+            (long long)sz[i] + nw[i] < (long long)bound_of(a, lab[i])) {
+            c.conn = conn[i];
+            c.tie = hash3(a.tie_seed, (uint32_t)v[i], (uint32_t)lab[i]);
+            c.lab = lab[i];
+            c.size = sz[i];
+        }
+        reduce_cand<G>(c);
+        r[i] = (valid[i] && gl == 0) ? propose<M>(a, v[i], own[i], nw[i], conn_own[i], c)
+                                     : Contrib{-1, 0, 0ull, false};
+    }
     // The block's proposals are combined per (target, kind) in an LDS hash before the atomics:
     // many low-degree nodes propose the same (hub) label, and one atomic per distinct target
     // keeps them from serialising on it.
-    constexpr int NB = kBlock / G;
-    constexpr int HC = 2 * NB;  // slots (load <= 1/2)
-    __shared__ int h_tag[HC];   // (label << 1) | departure, -1: empty
+    constexpr int HC = 2 * IT * NB;  // slots (load <= 1/2)
+    __shared__ int h_tag[HC];        // (label << 1) | departure, -1: empty
     __shared__ int h_w[HC];
     __shared__ unsigned long long h_key[HC];
-    if (threadIdx.x < HC) {
-        h_tag[threadIdx.x] = -1;
-        h_w[threadIdx.x] = 0;
-        h_key[threadIdx.x] = 0ull;
+    for (int i = threadIdx.x; i < HC; i += kBlock) {
+        h_tag[i] = -1;
+        h_w[i] = 0;
+        h_key[i] = 0ull;
     }
-    const Contrib r = (valid && gl == 0) ? propose<M>(a, v, own, nw, conn_own, c)
-                                         : Contrib{-1, 0, 0ull, false};
     __syncthreads();
-    if (r.label >= 0) {
-        const int tag = (r.label << 1) | (r.departure ? 1 : 0);
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        if (r[i].label < 0) continue;
+        const int tag = (r[i].label << 1) | (r[i].departure ? 1 : 0);
         uint32_t h = mix32((uint32_t)tag) & (uint32_t)(HC - 1);
         while (true) {
             const int old = atomicCAS(&h_tag[h], -1, tag);
             if (old == -1 || old == tag) {
-                atomicAdd(&h_w[h], r.w);
-                atomicMax(&h_key[h], r.key);
+                atomicAdd(&h_w[h], r[i].w);
+                atomicMax(&h_key[h], r[i].key);
                 break;
             }
             h = (h + 1) & (uint32_t)(HC - 1);
         }
     }
     __syncthreads();
-    if (threadIdx.x < HC && h_tag[threadIdx.x] >= 0) {
-        const int tag = h_tag[threadIdx.x];
-        add_contrib(a, Contrib{tag >> 1, h_w[threadIdx.x], h_key[threadIdx.x], (tag & 1) != 0});
+    for (int i = threadIdx.x; i < HC; i += kBlock) {
+        const int tag = h_tag[i];
+        if (tag >= 0) add_contrib(a, Contrib{tag >> 1, h_w[i], h_key[i], (tag & 1) != 0});
     }
     if (M == LpMode::Refine) {
         __shared__ unsigned long long tsum;
         if (threadIdx.x == 0) tsum = 0;
         __syncthreads();
-        if (valid && gl == 0) atomicAdd(&tsum, (unsigned long long)deg);
+        unsigned long long mine = 0;
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            if (valid[i] && gl == 0) mine += (unsigned long long)deg[i];
+        if (mine) atomicAdd(&tsum, mine);
         __syncthreads();
         if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[vb & 63], tsum);
     }
@@ -1565,20 +1601,20 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? ledges : 0, s);
                 switch (b) {
                     case 0:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(std::min(grid_for((long long)cnt * 8, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(std::min(grid_for((long long)cnt * 8, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 1:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(std::min(grid_for((long long)cnt * 16, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(std::min(grid_for((long long)cnt * 16, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 2:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(std::min(grid_for((long long)cnt * 32, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(std::min(grid_for((long long)cnt * 32, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 3:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(std::min(grid_for((long long)cnt * 64, kBlock), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
+                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(std::min(grid_for((long long)cnt * 64, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
                         break;
                     case 4:
                         if (cl) {
