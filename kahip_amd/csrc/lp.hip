@@ -52,6 +52,9 @@ constexpr int kFinCap = 16384;     // LDS hash slots of hub_bucket_merge (128 KB
 constexpr int kFinLds = 8192;      // buckets up to this many entries merge in LDS
 constexpr int kMaxBuckets = 2048;  // buckets per hub: min(kBucketsPerChunk x chunks, kMaxBuckets)
 constexpr int kBucketsPerChunk = 4;  // ~1000 entries per (hub, bucket): the small merge kernel takes most
+constexpr int kTinyThreads = 256;    // tiny bucket merge: <= kTinyMax entries in 16 KB of LDS
+constexpr int kTinyCap = 2048;
+constexpr int kTinyMax = 1024;
 constexpr int kSmallThreads = 512;   // small bucket merge: <= kSmallMax entries in 32 KB of LDS
 constexpr int kSmallCap = 4096;
 constexpr int kSmallMax = 2048;
@@ -718,24 +721,27 @@ __global__ __launch_bounds__(256) void hub_scatter(
 // Merge of one (multi-chunk hub, bucket) task: the bucket's entries of all chunks of the hub are
 // summed in an LDS hash (or, past kFinLds entries, in the hub's global table, filtering by
 // bucket); the bucket's best candidate and, for the bucket of the own label, conn(own) are left
-// in bres / bown. SMALL: 256 threads and 16 KB of LDS for buckets of <= kSmallMax entries;
-// larger buckets are queued for the large variant (1024 threads, 128 KB of LDS), which
-// grid-strides over that queue.
-template <LpMode M, bool SMALL>
-__global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucket_merge(
+// in bres / bown. Three sizes: LEVEL 0 (256 threads, 16 KB of LDS, eight per CU) takes every
+// task of the sub-round and queues those of more than kTinyMax entries for LEVEL 1 (512 threads,
+// 32 KB), which queues those of more than kSmallMax for LEVEL 2 (1024 threads, 128 KB); the
+// queued levels grid-stride over their queue.
+template <LpMode M, int LEVEL>
+__global__ __launch_bounds__(LEVEL == 0 ? kTinyThreads : (LEVEL == 1 ? kSmallThreads : kFinThreads))
+void hub_bucket_merge(
     LpArgs a, int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
     const int* __restrict__ posT, const int2* __restrict__ bucketed,
     const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
-    int* __restrict__ bown, uint8_t* __restrict__ fb, int* __restrict__ big_tasks,
-    int* __restrict__ n_big_tasks) {
-    constexpr int THREADS = SMALL ? kSmallThreads : kFinThreads;
-    constexpr int CAP = SMALL ? kSmallCap : kFinCap;
+    int* __restrict__ bown, uint8_t* __restrict__ fb, const int* __restrict__ in_q,
+    const int* __restrict__ in_n, int* __restrict__ out_q, int* __restrict__ out_n) {
+    constexpr int THREADS = LEVEL == 0 ? kTinyThreads : (LEVEL == 1 ? kSmallThreads : kFinThreads);
+    constexpr int CAP = LEVEL == 0 ? kTinyCap : (LEVEL == 1 ? kSmallCap : kFinCap);
+    constexpr int MAXE = LEVEL == 0 ? kTinyMax : kSmallMax;  // queued beyond (levels 0, 1)
     __shared__ int K[CAP];
     __shared__ int V[CAP];
-    const int n_iter = SMALL ? n_tasks : *n_big_tasks;
+    const int n_iter = LEVEL == 0 ? n_tasks : *in_n;
     for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
-        const int task = SMALL ? it : big_tasks[it];
+        const int task = LEVEL == 0 ? it : in_q[it];
         const int4 tm = tmap[task];  // {first count index, one past the last, hub index, node}
         const int2 ts = tsub[task];  // {multi-chunk hub index, bucket | buckets << 16}
         const int mi = ts.x;
@@ -750,12 +756,12 @@ __global__ __launch_bounds__(SMALL ? kSmallThreads : kFinThreads) void hub_bucke
         const int r0 = posT[tm.x];
         const int r1 = posT[tm.y];
         const int total = r1 - r0;
-        if (SMALL && total > kSmallMax) {
-            if (threadIdx.x == 0) big_tasks[atomicAdd(n_big_tasks, 1)] = task;
+        if (LEVEL < 2 && total > MAXE) {
+            if (threadIdx.x == 0) out_q[atomicAdd(out_n, 1)] = task;
             __syncthreads();
             continue;
         }
-        const bool in_lds = SMALL || total <= kFinLds;
+        const bool in_lds = LEVEL < 2 || total <= kFinLds;
         const int deg = a.xadj[v + 1] - a.xadj[v];
         long long cap = CAP;
         int* TK = K;
@@ -1240,6 +1246,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     big_.reserve(g.n);
     n_big_.reserve(kMaxSub * kMaxRounds);
     n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
+    n_mid_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     trav_.reserve(64);
 
     fill_async(counts_.p, L + kMaxSub, 0, s);
@@ -1433,6 +1440,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     bres_.reserve(max_tasks);
     bown_.reserve(max_tasks);
     big_tasks_.reserve(max_tasks);
+    mid_tasks_.reserve(max_tasks);
     fb_.reserve(max_multi);
     fill_async(fb_.p, max_multi, (uint8_t)0, s);
     KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
@@ -1521,6 +1529,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     // no allocation here: run() may be captured into a hipGraph (prepare() sized everything)
     fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_big_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
+    fill_async(n_mid_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
     fill_async(trav_.p, 64, 0ull, s);
     if (!cl) {
@@ -1672,13 +1681,17 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, cm, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
+                            int* mt = mid_tasks_.p;
+                            int* nmt = n_mid_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, true>), dim3(nt), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, false>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

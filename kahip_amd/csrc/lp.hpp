@@ -98,7 +98,7 @@ private:
     DevBuf<int> ct_base_, cnt_t_, pos_t_;
     DevBuf<char> scan_tmp_;
     DevBuf<long long> offm_base_;
-    DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_;
+    DevBuf<int> offm_, task_prefix_, bown_, big_tasks_, n_big_tasks_, mid_tasks_, n_mid_tasks_;
     DevBuf<int4> bres_;
     DevBuf<int4> cmap_, tmap_, mmap_;  // hub-chunk, (hub, bucket) task, mid-size hub descriptors
     DevBuf<int> midpos_;
