@@ -470,7 +470,9 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     __shared__ int M[kGCap];
     __shared__ int bcur[kGMaxBuckets + 1];
     __shared__ int wsum[kGThreads / 64];
-    __shared__ int row;
+    __shared__ int sbeg[kGChunk];  // edge-space start (mpre) of the chunk's members
+    __shared__ int sst[kGChunk];   // their CSR row start
+    __shared__ int row, m0s, nms;
     const int chunk = blockIdx.x;
     if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
     for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
@@ -489,32 +491,55 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     const long long f_end = mpre[me];
     const long long f0 = f_row + (long long)c_idx * kGChunk;
     const long long f1 = min(f_end, f0 + kGChunk);
-    constexpr int PER = kGChunk / kGThreads;  // a contiguous run of PER member edges per thread
-    const long long a0 = f0 + (long long)threadIdx.x * PER;
-    int es[PER], qs[PER], ws[PER];
-#pragma unroll
-    for (int j = 0; j < PER; ++j) es[j] = -1;
-    if (a0 < f1) {
-        int lo = mb, hi = me;  // largest member index with mpre <= a0
+    // The members overlapping [f0, f1) are staged in LDS (one global binary search per chunk);
+    // thread t then takes the chunk's edges t, t + kGThreads, ... and finds each one's member by
+    // a binary search in LDS, so neighbouring threads load neighbouring edges. (Members of degree
+    // 0 share their start with the next member; more than kGChunk members in a chunk can only
+    // come from those, and such a chunk searches the member prefix in global memory instead.)
+    constexpr int PER = kGChunk / kGThreads;
+    if (threadIdx.x == 0) {
+        int lo = mb, hi = me;  // largest member index with mpre <= f0
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if ((long long)mpre[mid] <= a0) lo = mid; else hi = mid;
+            if ((long long)mpre[mid] <= f0) lo = mid; else hi = mid;
         }
-        int mi = lo;
-        int u = members[mi];
-        long long mstart = mpre[mi];
-        long long mend = mstart + (xadj[u + 1] - xadj[u]);
+        int lo2 = lo, hi2 = me;  // largest member index with mpre < f1
+        while (hi2 - lo2 > 1) {
+            const int mid = (lo2 + hi2) >> 1;
+            if ((long long)mpre[mid] < f1) lo2 = mid; else hi2 = mid;
+        }
+        m0s = lo;
+        nms = lo2 - lo + 1;
+    }
+    __syncthreads();
+    const int m0 = m0s, nm = nms;
+    const bool staged = nm <= kGChunk;  // uniform
+    if (staged)
+        for (int i = threadIdx.x; i < nm; i += kGThreads) {
+            sbeg[i] = mpre[m0 + i];
+            sst[i] = xadj[members[m0 + i]];
+        }
+    __syncthreads();
+    int es[PER], qs[PER], ws[PER];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const long long f = a0 + j;
-            if (f >= f1) break;
-            while (f >= mend) {
-                ++mi;
-                u = members[mi];
-                mstart = mpre[mi];
-                mend = mstart + (xadj[u + 1] - xadj[u]);
+    for (int j = 0; j < PER; ++j) {
+        const long long f = f0 + threadIdx.x + (long long)j * kGThreads;
+        es[j] = -1;
+        if (f >= f1) continue;
+        if (staged) {
+            int lo = 0, hi = nm - 1;  // last staged member starting at or before f
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((long long)sbeg[mid] <= f) lo = mid; else hi = mid - 1;
             }
-            es[j] = xadj[u] + (int)(f - mstart);
+            es[j] = sst[lo] + (int)(f - sbeg[lo]);
+        } else {
+            int lo = mb, hi = me;  // largest member index with mpre <= f
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if ((long long)mpre[mid] <= f) lo = mid; else hi = mid;
+            }
+            es[j] = xadj[members[lo]] + (int)(f - mpre[lo]);
         }
     }
 #pragma unroll
