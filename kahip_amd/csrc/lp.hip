@@ -710,11 +710,22 @@ __global__ __launch_bounds__(256) void hub_scatter(
     const int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
     const int* pos = posT + ct_base[gi] + c_idx;
     const int2* src = partials + (long long)chunk * kHubChunk;
+    // destination of entry i of bucket b: pos[b * nch] + (i - offs[b]), staged per bucket
+    __shared__ int sbase[kMaxBuckets];
+    for (int b = threadIdx.x; b < P; b += 256) sbase[b] = pos[(long long)b * nch] - offs[b];
     const int total = offs[P];
-    for (int i = threadIdx.x; i < total; i += 256) {
-        const int2 kv = src[i];
-        const int b = bucket_of(kv.x, P);
-        bucketed[pos[(long long)b * nch] + (i - offs[b])] = kv;
+    __syncthreads();
+    constexpr int U = 4;  // entries per thread in flight
+    for (int i0 = threadIdx.x; i0 < total; i0 += 256 * U) {
+        int2 kv[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int i = i0 + 256 * j;
+            kv[j] = i < total ? src[i] : make_int2(-1, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+            if (kv[j].x >= 0) bucketed[sbase[bucket_of(kv[j].x, P)] + i0 + 256 * j] = kv[j];
     }
 }
 
