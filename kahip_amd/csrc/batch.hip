@@ -147,22 +147,7 @@ void run_lp(LpWorkspace& lp, bool use_graph, LpMode mode, const DGraph& g, int* 
         lp.run(mode, g, label, size, 0, rounds, seed, s, nullptr, phase, lbound);
         return;
     }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    KH_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-        lp.run(mode, g, label, size, 0, rounds, seed, s, nullptr, phase, lbound);
-    } catch (...) {
-        (void)hipStreamEndCapture(s, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        throw;
-    }
-    KH_CHECK(hipStreamEndCapture(s, &graph));
-    KH_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    KH_CHECK(hipGraphLaunch(exec, s));
-    KH_CHECK(hipStreamSynchronize(s));
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    run_lp_graph(lp, mode, g, label, size, 0, rounds, seed, s, phase, lbound);
 }
 
 }  // namespace

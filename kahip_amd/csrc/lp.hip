@@ -1731,6 +1731,27 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     KH_CHECK(hipGetLastError());
 }
 
+void run_lp_graph(LpWorkspace& lp, LpMode mode, const DGraph& g, int* label, int* size,
+                  int bound, int rounds, uint32_t seed, hipStream_t s, const char* phase,
+                  const int* lbound) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    KH_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        lp.run(mode, g, label, size, bound, rounds, seed, s, nullptr, phase, lbound);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    KH_CHECK(hipStreamEndCapture(s, &graph));
+    KH_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    KH_CHECK(hipGraphLaunch(exec, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+}
+
 void LpWorkspace::read_moves(int rounds, int* out, hipStream_t s) {
     if (rounds <= 0) return;
     KH_CHECK(hipMemcpyAsync(out, moves_.p, sizeof(int) * rounds, hipMemcpyDeviceToHost, s));
