@@ -230,14 +230,8 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             rated.adjwgt = score.p;
         }
         lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, cseed, p.bins, s);
-        // unprofiled, the clustering rounds run as one captured graph as well; profiled, they
-        // run eagerly so that every launch is timed
-        if (prof)
-            lp.run(LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds, cseed, s,
-                   prof, "lp_cluster");
-        else
-            run_lp_graph(lp, LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds,
-                         cseed, s, "lp_cluster");
+        lp.run(LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
+               "lp_cluster");
         long long edges_round = 0;
         for (int i = 0; i < p.subrounds * kNumBins; ++i) edges_round += lp.host_edges[i];
         S.edges_cluster += edges_round * p.cluster_rounds;
@@ -248,12 +242,8 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             elabel.reserve(fine.n);
             LpWorkspace::init_cluster_labels(fine, elabel.p, size.p, s);
             lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, eseed, p.bins, s);
-            if (prof)
-                lp.run(LpMode::Cluster, rated, elabel.p, size.p, eb, p.cluster_rounds, eseed, s,
-                       prof, "lp_cluster");
-            else
-                run_lp_graph(lp, LpMode::Cluster, rated, elabel.p, size.p, eb, p.cluster_rounds,
-                             eseed, s, "lp_cluster");
+            lp.run(LpMode::Cluster, rated, elabel.p, size.p, eb, p.cluster_rounds, eseed, s, prof,
+                   "lp_cluster");
             S.edges_cluster += edges_round * p.cluster_rounds;
             overlay(fine.n, label.p, elabel.p, s);
         }
