@@ -38,6 +38,8 @@ typedef struct {
     int ensemble;        /* LP clusterings per level, overlaid (1 = single clustering; the
                             STRONGSOCIAL preset uses 18/17/15/7/3 for k <= 3/7/15/31/more) */
     int ip_tries;        /* BFS-grown starts per bisection of the initial partitioner (32) */
+    int fm_rounds;       /* > 0: CPU 2-way FM over adjacent block pairs on the finest level after
+                            the LP refinement, at most this many passes (0 = off) */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);
@@ -69,6 +71,8 @@ typedef struct {
     long long max_block_weight;
     double ms_rating; /* rating time, part of ms_coarsen */
     int reordered;    /* the input was renumbered by degree */
+    double ms_fm;     /* CPU pairwise FM on the finest level (kahip_params.fm_rounds > 0) */
+    long long fm_gain; /* its cut reduction */
 } kahip_stats;
 
 void kahip_default_params(kahip_params* p);
@@ -125,6 +129,14 @@ int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_pa
 int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
                             const int* adjwgt, int k, long long block_bound, int repetitions,
                             unsigned long long seed, int* part);
+
+/* CPU refinement of a k-way partition by 2-way FM on the pairs of adjacent blocks (replaces
+ * the quotient-graph refinement of lib/partition/uncoarsening/refinement/quotient_graph_refinement/
+ * quotient_graph_refinement.cpp:48-130 with two_way_fm.cpp). part[n] in [0, k) is refined in
+ * place; *gain (nullable) receives the cut reduction. No GPU. */
+int kahip_fm_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, long long block_bound, int rounds,
+                    unsigned long long seed, int* part, long long* gain);
 
 /* Learned edge rating (no reference counterpart; the reference rates edges in
  * lib/partition/coarsening/edge_rating/edge_ratings.cpp:171-183 and does not use the rating on

@@ -26,6 +26,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libkahip.so")
 
 FAST, ECO, STRONG, FASTSOCIAL, ECOSOCIAL, STRONGSOCIAL = range(6)
+# presets whose reference refinement is FM-based: kaffpa adds the CPU pairwise FM on the finest
+# level (kahip_params.fm_rounds) after the LP refinement
+FM_MODES = (ECO, STRONG, ECOSOCIAL, STRONGSOCIAL)
+PRESET_FM_ROUNDS = 3
 MAX_LEVELS = 64
 RATING_NONE, RATING_EXPANSION_STAR2, RATING_LEARNED = 0, 1, 2
 MLP_FEATURES, MLP_HIDDEN, MLP_PARAMS = 8, 64, 4801
@@ -47,7 +51,7 @@ class Params(C.Structure):
                 ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
                 ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
                 ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int),
-                ("ensemble", C.c_int), ("ip_tries", C.c_int)]
+                ("ensemble", C.c_int), ("ip_tries", C.c_int), ("fm_rounds", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -70,7 +74,7 @@ class Stats(C.Structure):
                 ("edges_cluster", C.c_longlong), ("edges_contract", C.c_longlong),
                 ("edges_refine", C.c_longlong), ("edges_cut", C.c_longlong),
                 ("max_block_weight", C.c_longlong), ("ms_rating", C.c_double),
-                ("reordered", C.c_int)]
+                ("reordered", C.c_int), ("ms_fm", C.c_double), ("fm_gain", C.c_longlong)]
 
     def as_dict(self):
         L = self.levels
@@ -84,7 +88,7 @@ class Stats(C.Structure):
             "edges_cluster": int(self.edges_cluster), "edges_contract": int(self.edges_contract),
             "edges_refine": int(self.edges_refine), "edges_cut": int(self.edges_cut),
             "max_block_weight": int(self.max_block_weight), "ms_rating": self.ms_rating,
-            "reordered": bool(self.reordered),
+            "reordered": bool(self.reordered), "ms_fm": self.ms_fm, "fm_gain": int(self.fm_gain),
         }
 
 
@@ -98,6 +102,7 @@ EXPORTS = [
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
     "kahip_mlp_train", "kahip_batch_from_host", "kahip_batch_generate", "kahip_batch_info",
     "kahip_batch_to_host", "kahip_batch_free", "kahip_batch_partition", "kahip_autotune_bins",
+    "kahip_fm_refine",
 ]
 
 _lib = None
@@ -154,6 +159,8 @@ def lib():
     L.kahip_project.argtypes = [C.c_int, _i32p, C.c_int, _i32p, _i32p]
     L.kahip_initial_partition.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
                                           C.c_int, C.c_ulonglong, _i32p]
+    L.kahip_fm_refine.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong, C.c_int,
+                                  C.c_ulonglong, _i32p, C.POINTER(C.c_longlong)]
     L.kahip_edge_features.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_uint, _f32p]
     L.kahip_graph_edge_features.argtypes = [G, C.c_int, C.c_uint, _f32p]
     L.kahip_mlp_forward.argtypes = [C.c_longlong, _f32p, _f32p, _f32p]
@@ -236,7 +243,8 @@ def kaffpa(vwgt, xadj, adjcwgt, adjncy, nparts, imbalance, suppress_output, seed
     part = np.zeros(max(n, 1), np.int32)
     cut = C.c_int(0)
     p = default_params(k=int(nparts), imbalance=float(imbalance), seed=int(seed),
-                       ensemble=ensemble_size(int(nparts)) if mode == STRONGSOCIAL else 1)
+                       ensemble=ensemble_size(int(nparts)) if mode == STRONGSOCIAL else 1,
+                       fm_rounds=PRESET_FM_ROUNDS if mode in FM_MODES else 0)
     _keep = (vw, ew)
     _check(lib().kaffpa_gpu_ex(n, _p(vw), xadj, _p(ew), adjncy, C.byref(p), C.byref(cut), part,
                                None))
@@ -420,6 +428,25 @@ def initial_partition(xadj, adjncy, vwgt, adjwgt, k, block_bound, repetitions=-1
                                          _p(vwgt), _p(adjwgt), k, int(block_bound), repetitions,
                                          seed, out))
     return out[:n]
+
+
+def fm_refine(xadj, adjncy, vwgt, adjwgt, k, block_bound, part, rounds=3, seed=0):
+    """CPU 2-way FM over the pairs of adjacent blocks (no GPU): (refined part, cut reduction)."""
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    out = np.array(part, np.int32, copy=True)
+    if n == 0:
+        return out, 0
+    gain = C.c_longlong(0)
+    _check(lib().kahip_fm_refine(n, xadj, adjncy if len(adjncy) else np.zeros(1, np.int32),
+                                 _p(vwgt), _p(adjwgt), int(k), int(block_bound), int(rounds),
+                                 int(seed) & 0xFFFFFFFFFFFFFFFF, out, C.byref(gain)))
+    return out, int(gain.value)
+
+
+def fm_seed(seed):
+    """Seed of the finest-level pairwise FM of a partition with this seed."""
+    return (((int(seed) & 0xFFFFFFFF) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF) ^ 0xF3F3F3F3
 
 
 # ----------------------------------------------------------------------------- edge ratings

@@ -14,6 +14,7 @@
 
 #include "batch.hpp"
 #include "driver.hpp"
+#include "fm_refine.hpp"
 #include "gen.hpp"
 #include "kahip_amd.h"
 
@@ -72,6 +73,8 @@ PartitionParams to_params(const kahip_params* q) {
     p.ensemble = q->ensemble;
     p.ip_tries = q->ip_tries;
     KH_REQUIRE(p.ip_tries >= 1, "ip_tries must be >= 1");
+    p.fm_rounds = q->fm_rounds;
+    KH_REQUIRE(p.fm_rounds >= 0, "fm_rounds must be >= 0");
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
@@ -134,6 +137,8 @@ void fill_stats(const PartitionStats& s, kahip_stats* o) {
     o->max_block_weight = s.max_block_weight;
     o->ms_rating = s.ms_rating;
     o->reordered = s.reordered ? 1 : 0;
+    o->ms_fm = s.ms_fm;
+    o->fm_gain = s.fm_gain;
 }
 
 // Greedy CPU rebalancing for kaffpa_balance(perfectly_balance): while a block exceeds `bound`,
@@ -244,6 +249,7 @@ void kahip_default_params(kahip_params* p) {
     p->reorder = d.reorder;
     p->ensemble = d.ensemble;
     p->ip_tries = d.ip_tries;
+    p->fm_rounds = d.fm_rounds;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }
@@ -262,9 +268,12 @@ void kaffpa(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts
     p.k = *nparts;
     p.imbalance = *imbalance;
     p.seed = seed;
-    // STRONGSOCIAL clusters with ensembles (app/configuration.h:584-591); its FM refinement is
-    // not part of this library, which refines with LP in every mode
+    // STRONGSOCIAL clusters with ensembles (app/configuration.h:584-591); the presets whose
+    // reference refinement is FM-based (ECO, STRONG, ECOSOCIAL, STRONGSOCIAL) add the CPU
+    // pairwise FM on the finest level after the LP refinement
     if (mode == STRONGSOCIAL) p.ensemble = ensemble_size_for(p.k);
+    if (mode == ECO || mode == STRONG || mode == ECOSOCIAL || mode == STRONGSOCIAL)
+        p.fm_rounds = kPresetFmRounds;
     if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
         report("kaffpa", edgecut);
 }
@@ -587,6 +596,34 @@ int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int
         std::vector<int> out;
         initial_partition(g, p, out);
         std::copy(out.begin(), out.end(), part);
+    });
+}
+
+int kahip_fm_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, long long block_bound, int rounds,
+                    unsigned long long seed, int* part, long long* gain) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(k >= 1, "k must be >= 1");
+        KH_REQUIRE(rounds >= 0, "rounds must be >= 0");
+        HostGraph g;
+        g.n = n;
+        g.xadj.assign(xadj, xadj + n + 1);
+        g.adjncy.assign(adjncy, adjncy + xadj[n]);
+        g.adjwgt.resize(xadj[n]);
+        g.vwgt.resize(n);
+        for (int e = 0; e < xadj[n]; ++e) g.adjwgt[e] = adjwgt ? adjwgt[e] : 1;
+        for (int v = 0; v < n; ++v) g.vwgt[v] = vwgt ? vwgt[v] : 1;
+        std::vector<int> pv(part, part + n);
+        for (int v = 0; v < n; ++v) KH_REQUIRE(pv[v] >= 0 && pv[v] < k, "block id out of range");
+        FmRefineParams fp;
+        fp.k = k;
+        fp.block_bound = block_bound;
+        fp.rounds = rounds;
+        fp.seed = seed;
+        const long long gn = fm_refine_pairs(g, fp, pv);
+        std::copy(pv.begin(), pv.end(), part);
+        if (gain) *gain = gn;
     });
 }
 

@@ -9,6 +9,7 @@
 // uncoarsening.cpp:65,84-85). The edge cut of the finest level is computed on the GPU.
 
 #include "driver.hpp"
+#include "fm_refine.hpp"
 
 #include "reorder.hpp"
 
@@ -317,6 +318,27 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     KH_CHECK(hipEventRecord(ev_refine.b, s));
     KH_CHECK(hipStreamSynchronize(s));
     S.ms_refine = ev_refine.ms();
+
+    // ------------------------------------------------------------------ CPU pairwise FM
+    if (p.fm_rounds > 0 && g0.n > 0) {
+        const auto t1 = std::chrono::steady_clock::now();
+        HostGraph h0;
+        download_graph(g0, h0, s);
+        std::vector<int> hp(g0.n);
+        KH_CHECK(hipMemcpyAsync(hp.data(), part_c.p, sizeof(int) * g0.n, hipMemcpyDeviceToHost,
+                                s));
+        KH_CHECK(hipStreamSynchronize(s));
+        FmRefineParams fp;
+        fp.k = p.k;
+        fp.block_bound = U;
+        fp.rounds = p.fm_rounds;
+        fp.seed = fm_seed(p.seed);
+        S.fm_gain = fm_refine_pairs(h0, fp, hp);
+        KH_CHECK(hipMemcpyAsync(part_c.p, hp.data(), sizeof(int) * g0.n, hipMemcpyHostToDevice,
+                                s));
+        KH_CHECK(hipStreamSynchronize(s));
+        S.ms_fm = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    }
 
     // ------------------------------------------------------------------ metrics
     EventPair ev_cut;

@@ -30,6 +30,7 @@ struct PartitionParams {
     int ip_repetitions = -1;   // -1: the reference's count (initial_partitioning.cpp:69-81)
     int ip_threads = 0;
     int ip_tries = 32;         // BFS-grown starts per bisection of the initial partitioner
+    int fm_rounds = 0;         // > 0: CPU pairwise FM on the finest level after LP refinement
     BinConfig bins;
     bool profile = false;
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels
@@ -55,6 +56,8 @@ struct PartitionStats {
     double ms_coarsen = 0, ms_initial = 0, ms_refine = 0, ms_cut = 0;
     double ms_rating = 0;  // part of ms_coarsen
     bool reordered = false;
+    double ms_fm = 0;      // CPU pairwise FM on the finest level
+    long long fm_gain = 0;
     long long edges_cluster = 0, edges_contract = 0, edges_refine = 0, edges_cut = 0;
     long long max_block_weight = 0;
 };
@@ -88,6 +91,11 @@ unsigned upper_bound_for(long long total_weight, int k, double imbalance);
 unsigned cluster_bound_for(unsigned upper, int factor);
 // initial_partitioning.cpp:69-81 for the FASTSOCIAL preset (16 repetitions, minipreps 4).
 int ip_repetitions_for(int k);
+constexpr int kPresetFmRounds = 3;  // pairwise FM passes of the FM-based kaffpa presets
+// seed of the finest-level pairwise FM: (uint32 seed) x 0x9E3779B97F4A7C15 ^ 0xF3F3F3F3
+inline uint64_t fm_seed(int seed) {
+    return ((uint64_t)(uint32_t)seed * 0x9E3779B97F4A7C15ULL) ^ 0xF3F3F3F3ULL;
+}
 
 class Engine {
 public:

@@ -69,30 +69,34 @@ def overlay(a, b):
 
 def mirror_partition(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0, cluster_rounds=10,
                      refine_rounds=25, subrounds=4, cluster_factor=18, stop_factor=5000,
-                     ip=None, scores=None, rating_levels=64, reorder=-1, ensemble=1):
+                     ip=None, scores=None, rating_levels=64, reorder=-1, ensemble=1, fm_rounds=0):
     """As mirror_partition_plain, after the library's degree renumbering of skewed inputs
     (reorder: -1 automatic, 0 off, 1 on); the partition is returned in the input numbering."""
     if reorder_wanted(xadj, reorder):
         g, inv = reorder_by_degree(xadj, adjncy, vwgt, adjwgt)
         part, info = mirror_partition_plain(*g, k, imbalance, seed, cluster_rounds,
                                             refine_rounds, subrounds, cluster_factor,
-                                            stop_factor, ip, scores, rating_levels, ensemble)
+                                            stop_factor, ip, scores, rating_levels, ensemble,
+                                            fm_rounds)
         info["reordered"] = True
         return part[inv], info
     part, info = mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance, seed,
                                         cluster_rounds, refine_rounds, subrounds, cluster_factor,
-                                        stop_factor, ip, scores, rating_levels, ensemble)
+                                        stop_factor, ip, scores, rating_levels, ensemble,
+                                        fm_rounds)
     info["reordered"] = False
     return part, info
 
 
 def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0,
                            cluster_rounds=10, refine_rounds=25, subrounds=4, cluster_factor=18,
-                           stop_factor=5000, ip=None, scores=None, rating_levels=64, ensemble=1):
+                           stop_factor=5000, ip=None, scores=None, rating_levels=64, ensemble=1,
+                           fm_rounds=0):
     """Returns (part, info). `ip(xadj, adjncy, vwgt, adjwgt, k, U, reps, seed)` partitions the
     coarsest graph (the library's kahip_amd.initial_partition by default). `scores(xadj, adjncy,
     vwgt, adjwgt, B, rating_seed)` (optional) gives the int32 edge scores that replace the edge
-    weights in the clustering of levels < rating_levels."""
+    weights in the clustering of levels < rating_levels. fm_rounds > 0 applies the library's CPU
+    pairwise FM (kahip_amd.fm_refine) to the finest level at the end, as the pipeline does."""
     if ip is None:
         import kahip_amd
         ip = kahip_amd.initial_partition
@@ -135,6 +139,11 @@ def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0
             part = O.project(maps[l], part)
         rs = O.hash3(seed, 0x4E, l)
         part, _ = O.lp_refine_sync(x, a, vw, ew, k, U, refine_rounds, subrounds, rs, part)
+    if fm_rounds > 0 and n > 0:
+        import kahip_amd
+        x, a, vw, ew = levels[0]
+        part, _ = kahip_amd.fm_refine(x, a, vw, ew, k, U, part, rounds=fm_rounds,
+                                      seed=kahip_amd.fm_seed(seed))
     info = {"U": U, "B": B, "levels": level,
             "level_n": [len(lv[0]) - 1 for lv in levels], "initial": initial,
             "coarsest": levels[level]}

@@ -285,8 +285,32 @@ def test_pipeline_ensemble_matches_mirror(G, name, k, seed, ens):
 def test_kaffpa_strongsocial_uses_ensembles(G):
     x, a, vw, ew = G["rgg15"]
     cut, blocks = K.kaffpa(vw, x, ew, a, 16, 0.03, True, 3, K.STRONGSOCIAL)
-    part, st = K.partition(x, a, vw, ew, k=16, imbalance=0.03, seed=3, ensemble=7)
+    part, st = K.partition(x, a, vw, ew, k=16, imbalance=0.03, seed=3, ensemble=7,
+                           fm_rounds=K.PRESET_FM_ROUNDS)
     assert np.array_equal(np.array(blocks), part) and cut == st["cut"]
+
+
+def test_kaffpa_eco_adds_pairwise_fm(G):
+    x, a, vw, ew = G["delaunay15"]
+    cut, blocks = K.kaffpa(vw, x, ew, a, 8, 0.03, True, 2, K.ECO)
+    part, st = K.partition(x, a, vw, ew, k=8, imbalance=0.03, seed=2,
+                           fm_rounds=K.PRESET_FM_ROUNDS)
+    assert np.array_equal(np.array(blocks), part) and cut == st["cut"]
+    _, plain = K.partition(x, a, vw, ew, k=8, imbalance=0.03, seed=2)
+    assert st["fm_gain"] == plain["cut"] - st["cut"] and st["fm_gain"] >= 0
+
+
+@pytest.mark.parametrize("name,k,seed", [("delaunay15", 8, 1), ("rgg15", 2, 2), ("rmat12", 16, 3),
+                                         ("weighted132", 4, 4)])
+def test_pipeline_fm_matches_mirror(G, name, k, seed):
+    """The finest-level pairwise FM (in the renumbered ids for skewed inputs) is bit-identical
+    to the mirror's call of the same CPU routine."""
+    x, a, vw, ew = G[name]
+    got, st = K.partition(x, a, vw, ew, k=k, imbalance=0.03, seed=seed, fm_rounds=3)
+    want, info = mirror_partition(x, a, vw, ew, k, 0.03, seed, fm_rounds=3)
+    assert np.array_equal(got, want)
+    assert st["cut"] == O.edge_cut(x, a, ew, want)
+    assert O.block_weights(vw, k, got).max() <= info["U"]
 
 
 def test_pipeline_deterministic(G):
