@@ -49,13 +49,71 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 __device__ inline float round_bf(float f) { return bf2f(f2bf(f)); }
 
 // ------------------------------------------------------------------ features
-__global__ void sketch_kernel(int n, const int* __restrict__ xadj, const int* __restrict__ adjncy,
-                              unsigned long long* __restrict__ sig) {
+// Sketch of a node with at most kSketchLight neighbours: one thread. Heavier nodes are appended
+// to heavy[1..] (heavy[0] counts them, zero on entry; capacity m2 / (kSketchLight + 1)) and
+// sketched by a whole workgroup each, so a hub's row is read by 256 lanes, not by one.
+constexpr int kSketchLight = 64;
+constexpr int kSketchHeavyGrid = 4096;
+
+__device__ inline unsigned long long sketch_bit(int y) {
+    return 1ull << (mix32((uint32_t)y) & 63u);
+}
+
+__global__ void sketch_light_kernel(int n, const int* __restrict__ xadj,
+                                    const int* __restrict__ adjncy,
+                                    unsigned long long* __restrict__ sig, int* __restrict__ heavy) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
+    const int b = xadj[u], end = xadj[u + 1];
+    if (end - b > kSketchLight) {
+        heavy[1 + atomicAdd(heavy, 1)] = u;
+        return;
+    }
     unsigned long long s = 0;
-    for (int e = xadj[u]; e < xadj[u + 1]; ++e) s |= 1ull << (mix32((uint32_t)adjncy[e]) & 63u);
+    for (int e = b; e < end; ++e) s |= sketch_bit(adjncy[e]);
     sig[u] = s;
+}
+
+__global__ __launch_bounds__(256) void sketch_heavy_kernel(const int* __restrict__ xadj,
+                                                           const int* __restrict__ adjncy,
+                                                           const int* __restrict__ heavy,
+                                                           unsigned long long* __restrict__ sig) {
+    __shared__ unsigned long long part[4];
+    const int count = heavy[0];
+    for (int i = blockIdx.x; i < count; i += gridDim.x) {
+        const int u = heavy[1 + i];
+        const int end = xadj[u + 1];
+        unsigned long long s = 0;
+#pragma unroll 4
+        for (int e = xadj[u] + (int)threadIdx.x; e < end; e += 256) s |= sketch_bit(adjncy[e]);
+        uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo |= __shfl_xor(lo, off);
+            hi |= __shfl_xor(hi, off);
+        }
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = ((unsigned long long)hi << 32) | lo;
+        __syncthreads();
+        if (threadIdx.x == 0) sig[u] = part[0] | part[1] | part[2] | part[3];
+        __syncthreads();
+    }
+}
+
+// Everything the features of an edge need from one endpoint, in one 32-byte record, so that the
+// neighbour side of an edge costs one random sector instead of five.
+struct __align__(32) NodeRec {
+    int deg, c;
+    unsigned long long sig;
+    int l0, l1, l2, pad;
+};
+
+__global__ void node_rec_kernel(int n, const int* __restrict__ xadj, const int* __restrict__ vwgt,
+                                const unsigned long long* __restrict__ sig,
+                                const int* __restrict__ l0, const int* __restrict__ l1,
+                                const int* __restrict__ l2, NodeRec* __restrict__ rec) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    rec[u] = NodeRec{xadj[u + 1] - xadj[u], vwgt[u], sig[u], l0[u], l1[u], l2[u], 0};
 }
 
 __device__ inline float lg(float x) { return __log2f(1.0f + x); }
@@ -65,9 +123,7 @@ __device__ inline float lg(float x) { return __log2f(1.0f + x); }
 template <class OUT>
 __global__ __launch_bounds__(256) void features_kernel(
     int n, long long m2, const int* __restrict__ xadj, const int* __restrict__ adjncy,
-    const int* __restrict__ adjwgt, const int* __restrict__ vwgt,
-    const unsigned long long* __restrict__ sig, const int* __restrict__ l0,
-    const int* __restrict__ l1, const int* __restrict__ l2, OUT* __restrict__ out) {
+    const int* __restrict__ adjwgt, const NodeRec* __restrict__ rec, OUT* __restrict__ out) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long e0 = t * 8;
     if (e0 >= m2) return;
@@ -78,18 +134,22 @@ __global__ __launch_bounds__(256) void features_kernel(
     }
     int u = lo;
     long long next = xadj[u + 1];
+    NodeRec ru = rec[u];
     const long long e1 = e0 + 8 < m2 ? e0 + 8 : m2;
     for (long long e = e0; e < e1; ++e) {
-        while (e >= next) {
-            ++u;
-            next = xadj[u + 1];
+        if (e >= next) {
+            while (e >= next) {
+                ++u;
+                next = xadj[u + 1];
+            }
+            ru = rec[u];
         }
-        const int v = adjncy[e];
-        const float du = (float)(xadj[u + 1] - xadj[u]), dv = (float)(xadj[v + 1] - xadj[v]);
+        const NodeRec rv = rec[adjncy[e]];
+        const float du = (float)ru.deg, dv = (float)rv.deg;
         const float w = (float)adjwgt[e];
-        const float cu = (float)vwgt[u], cv = (float)vwgt[v];
-        const float cn = (float)__popcll(sig[u] & sig[v]);
-        const float agree = (float)((l0[u] == l0[v]) + (l1[u] == l1[v]) + (l2[u] == l2[v]));
+        const float cu = (float)ru.c, cv = (float)rv.c;
+        const float cn = (float)__popcll(ru.sig & rv.sig);
+        const float agree = (float)((ru.l0 == rv.l0) + (ru.l1 == rv.l1) + (ru.l2 == rv.l2));
         const float es2 = (w * w) / fmaxf(cu * cv, 1.0f);
         const float f[kFeatures] = {lg(fminf(du, dv)), lg(fmaxf(du, dv)), lg(w),
                                     lg(fminf(cu, cv)), lg(fmaxf(cu, cv)), cn * 0.125f, agree,
@@ -425,33 +485,48 @@ void pack_weights(const MlpWeights& w, std::vector<uint8_t>& out) {
 }  // namespace
 
 void RatingWorkspace::sweeps_and_sketch(const DGraph& g, int bound, uint32_t seed, hipStream_t s,
-                                        LpWorkspace& lp) {
+                                        LpWorkspace& lp, Profiler* prof) {
     sig_.reserve(g.n);
     lab_.reserve(3LL * g.n);
     size_.reserve(g.n);
-    if (g.n > 0)
-        hipLaunchKernelGGL(sketch_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
-                           g.adjncy, sig_.p);
-    // three one-round sweeps with a single sub-round: the node lists do not depend on the seed
-    // then, so they are built once; the seeds only change the tie-breaking
-    BinConfig bins;
-    lp.prepare(LpMode::Cluster, g, g.n, 1, 0u, bins, s);
-    for (int i = 0; i < 3; ++i) {
-        int* lab = lab_.p + (long long)i * g.n;
-        const uint32_t sd = hash3(seed, 0xA6u, (uint32_t)i);
-        LpWorkspace::init_cluster_labels(g, lab, size_.p, s);
-        lp.run(LpMode::Cluster, g, lab, size_.p, bound, 1, sd, s, nullptr, "rating_sweep");
+    rec_.reserve((long long)g.n * (sizeof(NodeRec) / sizeof(int)));
+    if (g.n == 0) return;
+    {
+        ProfScope ps(prof, "rating_sketch", 0.0, g.m2, s);
+        heavy_.reserve(2 + g.m2 / (kSketchLight + 1));
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, heavy_.p);
+        hipLaunchKernelGGL(sketch_light_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n,
+                           g.xadj, g.adjncy, sig_.p, heavy_.p);
+        hipLaunchKernelGGL(sketch_heavy_kernel, dim3(kSketchHeavyGrid), dim3(256), 0, s, g.xadj,
+                           g.adjncy, heavy_.p, sig_.p);
     }
+    {
+        // three one-round sweeps with a single sub-round: the node lists do not depend on the
+        // seed then, so they are built once; the seeds only change the tie-breaking
+        ProfScope ps(prof, "rating_sweeps", 0.0, 3 * g.m2, s);
+        BinConfig bins;
+        lp.prepare(LpMode::Cluster, g, g.n, 1, 0u, bins, s);
+        for (int i = 0; i < 3; ++i) {
+            int* lab = lab_.p + (long long)i * g.n;
+            const uint32_t sd = hash3(seed, 0xA6u, (uint32_t)i);
+            LpWorkspace::init_cluster_labels(g, lab, size_.p, s);
+            lp.run(LpMode::Cluster, g, lab, size_.p, bound, 1, sd, s, nullptr, "rating_sweep");
+        }
+    }
+    hipLaunchKernelGGL(node_rec_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
+                       g.vwgt, sig_.p, lab_.p, lab_.p + g.n, lab_.p + 2LL * g.n,
+                       reinterpret_cast<NodeRec*>(rec_.p));
+    KH_CHECK(hipGetLastError());
 }
 
 void RatingWorkspace::features(const DGraph& g, int bound, uint32_t seed, float* out,
                                hipStream_t s, LpWorkspace& lp, Profiler* prof) {
+    sweeps_and_sketch(g, bound, seed, s, lp, prof);
     ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
-    sweeps_and_sketch(g, bound, seed, s, lp);
     if (g.m2 > 0)
         hipLaunchKernelGGL(features_kernel<float>, dim3(grid_for((g.m2 + 7) / 8, 256)), dim3(256),
-                           0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p, lab_.p,
-                           lab_.p + g.n, lab_.p + 2LL * g.n, out);
+                           0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt,
+                           reinterpret_cast<const NodeRec*>(rec_.p), out);
     KH_CHECK(hipGetLastError());
 }
 
@@ -508,12 +583,12 @@ void RatingWorkspace::scores(RatingMode mode, const DGraph& g, int bound, uint32
     // features rounded to BF16 (16 B per edge), then the MLP writes the quantised scores
     featb_.reserve(g.m2 * 8);
     upload_weights(*w, s);
+    sweeps_and_sketch(g, bound, seed, s, lp, prof);
     {
         ProfScope ps(prof, "rating_features", 0.0, g.m2, s);
-        sweeps_and_sketch(g, bound, seed, s, lp);
         hipLaunchKernelGGL(features_kernel<bf16x8>, dim3(grid_for((g.m2 + 7) / 8, 256)),
-                           dim3(256), 0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt, g.vwgt, sig_.p,
-                           lab_.p, lab_.p + g.n, lab_.p + 2LL * g.n,
+                           dim3(256), 0, s, g.n, g.m2, g.xadj, g.adjncy, g.adjwgt,
+                           reinterpret_cast<const NodeRec*>(rec_.p),
                            reinterpret_cast<bf16x8*>(featb_.p));
     }
     {

@@ -54,10 +54,10 @@ public:
 
 private:
     void sweeps_and_sketch(const DGraph& g, int bound, uint32_t seed, hipStream_t s,
-                           LpWorkspace& lp);
+                           LpWorkspace& lp, Profiler* prof);
     void upload_weights(const MlpWeights& w, hipStream_t s);
     DevBuf<unsigned long long> sig_;
-    DevBuf<int> lab_, size_, map_;
+    DevBuf<int> lab_, size_, map_, heavy_, rec_;
     DevBuf<float> feat_, logit_, partial_, loss_;
     DevBuf<uint16_t> wpack_, featb_;
 };

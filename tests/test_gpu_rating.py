@@ -35,6 +35,22 @@ def rand_params(seed, scale=1.0):
     return p.astype(np.float32)
 
 
+def hub_graph(n=6000):
+    """A ring plus two hubs of degree n - 1 and about n / 3: rows far longer than one thread's
+    share of the neighbourhood sketch, next to rows of degree 2-4."""
+    u = [np.arange(n), np.zeros(n - 1, np.int64), np.ones(n // 3 - 1, np.int64)]
+    v = [(np.arange(n) + 1) % n, np.arange(1, n), np.arange(1, n // 3) * 3 + 2]
+    u, v = np.concatenate(u), np.concatenate(v)
+    pairs = np.unique(np.stack([np.minimum(u, v), np.maximum(u, v)], 1), axis=0)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    uu = np.concatenate([pairs[:, 0], pairs[:, 1]])
+    vv = np.concatenate([pairs[:, 1], pairs[:, 0]])
+    order = np.lexsort((vv, uu))
+    xadj = np.zeros(n + 1, np.int64)
+    np.add.at(xadj, uu + 1, 1)
+    return np.cumsum(xadj).astype(np.int32), vv[order].astype(np.int32), None, None
+
+
 @pytest.fixture(scope="module")
 def graphs(rgg15, weighted132):
     return {
@@ -43,10 +59,11 @@ def graphs(rgg15, weighted132):
         "weighted132": weighted132,
         "rgg15": rgg15,
         "rmat12": K.gen_rmat(12, 16 << 12, 3),
+        "hubs": hub_graph(),
     }
 
 
-@pytest.mark.parametrize("name", ["manual", "er300w", "weighted132", "rgg15", "rmat12"])
+@pytest.mark.parametrize("name", ["manual", "er300w", "weighted132", "rgg15", "rmat12", "hubs"])
 def test_features_match_oracle(graphs, name):
     x, a, vw, ew = graphs[name]
     bound, seed = 40, 1234
