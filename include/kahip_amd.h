@@ -40,6 +40,9 @@ typedef struct {
     int ip_tries;        /* BFS-grown starts per bisection of the initial partitioner (32) */
     int fm_rounds;       /* > 0: CPU 2-way FM over adjacent block pairs on the finest level after
                             the LP refinement, at most this many passes (0 = off) */
+    int graph_refine;    /* 1: run each level's LP refinement rounds as one captured hipGraph;
+                            0 (default): direct launches, measured faster on every tested size;
+                            the partition does not depend on it */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);

@@ -51,7 +51,8 @@ class Params(C.Structure):
                 ("ip_repetitions", C.c_int), ("ip_threads", C.c_int), ("bins", C.c_int * 6),
                 ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
                 ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int),
-                ("ensemble", C.c_int), ("ip_tries", C.c_int), ("fm_rounds", C.c_int)]
+                ("ensemble", C.c_int), ("ip_tries", C.c_int), ("fm_rounds", C.c_int),
+                ("graph_refine", C.c_int)]
 
 
 class BatchStats(C.Structure):

@@ -75,6 +75,7 @@ PartitionParams to_params(const kahip_params* q) {
     KH_REQUIRE(p.ip_tries >= 1, "ip_tries must be >= 1");
     p.fm_rounds = q->fm_rounds;
     KH_REQUIRE(p.fm_rounds >= 0, "fm_rounds must be >= 0");
+    p.graph_refine = q->graph_refine;
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)
@@ -250,6 +251,7 @@ void kahip_default_params(kahip_params* p) {
     p->ensemble = d.ensemble;
     p->ip_tries = d.ip_tries;
     p->fm_rounds = d.fm_rounds;
+    p->graph_refine = d.graph_refine;
 }
 
 const char* kahip_last_error(void) { return g_last_error.c_str(); }

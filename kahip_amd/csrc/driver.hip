@@ -309,10 +309,13 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         compute_block_weights(gl, part_l, p.k, W.p, s);
         const uint32_t rseed = refine_seed(p.seed, l);
         lp.prepare(LpMode::Refine, gl, p.k, p.subrounds, rseed, p.bins, s);
-        // the refinement rounds are many small launches: one captured graph per level (the
-        // profiler times the clustering and contraction kernels, not these)
-        run_lp_graph(lp, LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s,
-                     "lp_refine");
+        // the profiler times the clustering and contraction kernels, not the refinement rounds
+        if (p.graph_refine > 0)
+            run_lp_graph(lp, LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s,
+                         "lp_refine");
+        else
+            lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, nullptr,
+                   "lp_refine");
         S.edges_refine += lp.traversed(s);
     }
     KH_CHECK(hipEventRecord(ev_refine.b, s));

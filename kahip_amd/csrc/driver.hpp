@@ -31,6 +31,10 @@ struct PartitionParams {
     int ip_threads = 0;
     int ip_tries = 32;         // BFS-grown starts per bisection of the initial partitioner
     int fm_rounds = 0;         // > 0: CPU pairwise FM on the finest level after LP refinement
+    // 1: each level's refinement rounds as one captured hipGraph. Off by default: capture and
+    // instantiation cost more than the launch overhead they save (RGG 2^15..2^22, R-MAT 2^20 and
+    // 2^24 measured).
+    int graph_refine = 0;
     BinConfig bins;
     bool profile = false;
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels

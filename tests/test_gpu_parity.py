@@ -320,6 +320,17 @@ def test_pipeline_deterministic(G):
     assert np.array_equal(p1, p2) and s1["cut"] == s2["cut"]
 
 
+@pytest.mark.parametrize("name,k,seed", [("rgg15", 8, 1), ("rmat12", 16, 3), ("star1500", 4, 0)])
+def test_graph_captured_refinement_matches_mirror(G, name, k, seed):
+    """graph_refine = 1 replays each level's refinement rounds from a captured hipGraph; the
+    partition is the mirror's, as with direct launches, and stays so on a second call."""
+    x, a, vw, ew = G[name]
+    want, _ = mirror_partition(x, a, vw, ew, k, 0.03, seed)
+    for _ in range(2):
+        part, _ = K.partition(x, a, vw, ew, k=k, imbalance=0.03, seed=seed, graph_refine=1)
+        assert np.array_equal(part, want)
+
+
 @pytest.mark.parametrize("scale,samples,seed", [(10, 16 << 10, 1), (12, 8 << 12, 2)])
 def test_rmat_generator_matches_numpy(scale, samples, seed):
     g = K.Graph.rmat(scale, samples, seed)
