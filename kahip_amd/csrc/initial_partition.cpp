@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <functional>
 #include <future>
 #include <numeric>
@@ -65,7 +66,8 @@ uint64_t stream_seed(uint64_t seed, uint64_t a, uint64_t b) {
 int lp_cluster(const HostGraph& g, long long bound, int iters, Rng& rng, std::vector<int>& label) {
     const int n = g.n;
     label.resize(n);
-    std::vector<long long> size(n);
+    // int32 sizes and connection sums: the library keeps total node and edge weight below 2^31
+    std::vector<int> size(n);
     std::vector<int> order(n);
     for (int v = 0; v < n; ++v) {
         label[v] = v;
@@ -73,7 +75,7 @@ int lp_cluster(const HostGraph& g, long long bound, int iters, Rng& rng, std::ve
         order[v] = v;
     }
     for (int i = n - 1; i > 0; --i) std::swap(order[i], order[rng.below(i + 1)]);
-    std::vector<long long> acc(n, 0);
+    std::vector<int> acc(n, 0);
     std::vector<int> touched;
     for (int it = 0; it < iters; ++it) {
         int moved = 0;
@@ -85,11 +87,12 @@ int lp_cluster(const HostGraph& g, long long bound, int iters, Rng& rng, std::ve
                 acc[c] += g.adjwgt[e];
             }
             int best = own;
-            long long bv = acc[own];
+            int bv = acc[own];
             for (int c : touched) {
-                if (c == own) continue;
-                if (size[c] + g.vwgt[v] >= bound) continue;
-                if (acc[c] > bv || (acc[c] == bv && rng.coin())) {
+                // the connection test first: it rejects most candidates without reading size[c]
+                if (c == own || acc[c] < bv) continue;
+                if ((long long)size[c] + g.vwgt[v] >= bound) continue;
+                if (acc[c] > bv || rng.coin()) {
                     best = c;
                     bv = acc[c];
                 }
@@ -128,7 +131,7 @@ HostGraph contract(const HostGraph& g, const std::vector<int>& label, int nc) {
     for (int p = 0; p < nc; ++p) start[p + 1] += start[p];
     std::vector<int> pos(start.begin(), start.end() - 1);
     for (int v = 0; v < g.n; ++v) members[pos[label[v]]++] = v;
-    std::vector<long long> acc(nc, 0);
+    std::vector<int> acc(nc, 0);
     std::vector<int> touched;
     c.adjncy.reserve(g.adjncy.size() / 2);
     c.adjwgt.reserve(g.adjncy.size() / 2);
@@ -144,7 +147,7 @@ HostGraph contract(const HostGraph& g, const std::vector<int>& label, int nc) {
         }
         for (int q : touched) {
             c.adjncy.push_back(q);
-            c.adjwgt.push_back((int)acc[q]);
+            c.adjwgt.push_back(acc[q]);
             acc[q] = 0;
         }
         touched.clear();
@@ -638,9 +641,20 @@ void initial_partition(const HostGraph& g, const IpParams& p, std::vector<int>& 
     if (threads <= 1) {  // the caller parallelises (e.g. over the graphs of a batch)
         for (int r = 0; r < reps; ++r) run_rep(r);
     } else {
+        // an exception (e.g. bad_alloc) is rethrown here, not left to end the process
+        std::vector<std::exception_ptr> err(reps);
         std::vector<std::thread> pool;
-        for (int r = 0; r < reps; ++r) pool.emplace_back(run_rep, r);
+        for (int r = 0; r < reps; ++r)
+            pool.emplace_back([&, r] {
+                try {
+                    run_rep(r);
+                } catch (...) {
+                    err[r] = std::current_exception();
+                }
+            });
         for (auto& t : pool) t.join();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
     }
     int best = 0;
     for (int r = 1; r < reps; ++r)
