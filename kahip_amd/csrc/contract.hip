@@ -189,8 +189,8 @@ __global__ __launch_bounds__(64) void rows_lds(
             }
 #pragma unroll
             for (int j = 0; j < kEpl; ++j) {
-                qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
-                ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+                qs[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjncy + es[j]) : 0;
+                ws[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjwgt + es[j]) : 0;
             }
 #pragma unroll
             for (int j = 0; j < kEpl; ++j)
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
                 }
 #pragma unroll
                 for (int j = 0; j < kEpl; ++j) {
-                    qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
-                    ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+                    qs[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjncy + es[j]) : 0;
+                    ws[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjwgt + es[j]) : 0;
                 }
 #pragma unroll
                 for (int j = 0; j < kEpl; ++j)
@@ -544,8 +544,8 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        qs[j] = es[j] >= 0 ? adjncy[es[j]] : 0;
-        ws[j] = es[j] >= 0 ? adjwgt[es[j]] : 0;
+        qs[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjncy + es[j]) : 0;
+        ws[j] = es[j] >= 0 ? __builtin_nontemporal_load(adjwgt + es[j]) : 0;
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j)

@@ -364,8 +364,9 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
 #pragma unroll
     for (int j = 0; j < EPL; ++j) {
         const int e = beg + lane + 64 * j;
-        labs[j] = e < beg + deg ? a.adjncy[e] : -1;
-        ws[j] = e < beg + deg ? a.adjwgt[e] : 0;
+        // a row of up to WCAP / 2 edges, read once per sub-round: streamed (non-temporal)
+        labs[j] = e < beg + deg ? __builtin_nontemporal_load(a.adjncy + e) : -1;
+        ws[j] = e < beg + deg ? __builtin_nontemporal_load(a.adjwgt + e) : 0;
     }
 #pragma unroll
     for (int j = 0; j < EPL; ++j)
@@ -473,8 +474,10 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
         const int e = e0 + threadIdx.x + kHubDenseThreads * j;
-        labs[j] = e < e1 ? a.adjncy[e] : -1;
-        ws[j] = e < e1 ? a.adjwgt[e] : 0;
+        // the hub's row is read once per sub-round: streamed (non-temporal), so the caches stay
+        // with the label gathers
+        labs[j] = e < e1 ? __builtin_nontemporal_load(a.adjncy + e) : -1;
+        ws[j] = e < e1 ? __builtin_nontemporal_load(a.adjwgt + e) : 0;
     }
 #pragma unroll
     for (int j = 0; j < EPT; ++j)
@@ -593,8 +596,10 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
         const int e = e0 + threadIdx.x + THREADS * j;
-        labs[j] = e < e1 ? a.adjncy[e] : -1;
-        ws[j] = e < e1 ? a.adjwgt[e] : 0;
+        // the hub's row is read once per sub-round: streamed (non-temporal), so the caches stay
+        // with the label gathers
+        labs[j] = e < e1 ? __builtin_nontemporal_load(a.adjncy + e) : -1;
+        ws[j] = e < e1 ? __builtin_nontemporal_load(a.adjwgt + e) : 0;
     }
 #pragma unroll
     for (int j = 0; j < EPT; ++j)
