@@ -216,8 +216,13 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        lab[i] = gl < deg[i] ? a.adjncy[beg[i] + gl] : -1;
-        w[i] = gl < deg[i] ? a.adjwgt[beg[i] + gl] : 0;
+        if constexpr (G >= 16) {  // rows of 9..64 entries: streamed (non-temporal)
+            lab[i] = gl < deg[i] ? __builtin_nontemporal_load(a.adjncy + beg[i] + gl) : -1;
+            w[i] = gl < deg[i] ? __builtin_nontemporal_load(a.adjwgt + beg[i] + gl) : 0;
+        } else {
+            lab[i] = gl < deg[i] ? a.adjncy[beg[i] + gl] : -1;
+            w[i] = gl < deg[i] ? a.adjwgt[beg[i] + gl] : 0;
+        }
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i)
