@@ -150,9 +150,10 @@ def measured_traffic(workload, kernel):
             t = json.load(fh)
     except (OSError, ValueError):
         return None
-    if t.get("profiler_scope") != kernel:
-        return None
-    return t.get("hbm_bytes_per_launch")
+    if t.get("profiler_scope") == kernel:
+        return t.get("hbm_bytes_per_launch")
+    # summaries may also list other kernels of the same run (close runners-up for dominance)
+    return (t.get("per_scope") or {}).get(kernel)
 
 
 def cpu_baseline(K, g, w, budget_s=30.0):
