@@ -282,6 +282,9 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     // many low-degree nodes propose the same (hub) label, and one atomic per distinct target
     // keeps them from serialising on it.
     constexpr int HC = 2 * IT * NB;  // slots (load <= 1/2)
+    // probed with a mask: a size that is not a power of two leaves slots unreachable and the
+    // probe loop below would never end
+    static_assert((HC & (HC - 1)) == 0, "kGroupItems x kBlock / G must be a power of two");
     __shared__ int h_tag[HC];        // (label << 1) | departure, -1: empty
     __shared__ int h_w[HC];
     __shared__ unsigned long long h_key[HC];
