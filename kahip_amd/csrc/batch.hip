@@ -143,11 +143,11 @@ struct Ev {
 // Runs the LP rounds directly or as one captured hipGraph (identical launches either way).
 void run_lp(LpWorkspace& lp, bool use_graph, LpMode mode, const DGraph& g, int* label, int* size,
             int rounds, uint32_t seed, hipStream_t s, const int* lbound, const char* phase) {
-    if (!use_graph) {
+    if (use_graph)
+        run_lp_graph(lp, mode, g, label, size, 0, rounds, seed, s, phase, lbound);
+    else
         lp.run(mode, g, label, size, 0, rounds, seed, s, nullptr, phase, lbound);
-        return;
-    }
-    run_lp_graph(lp, mode, g, label, size, 0, rounds, seed, s, phase, lbound);
+    lp.check(s);
 }
 
 }  // namespace
@@ -175,6 +175,7 @@ BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, double* ms) {
         lp.run(LpMode::Cluster, g, label.p, size.p, B, 2, seed, s, nullptr, "autotune");
         KH_CHECK(hipEventRecord(ev.b, s));
         KH_CHECK(hipStreamSynchronize(s));
+        lp.check(s);
         const float t = ev.ms();
         if (ms) ms[c] = t;
         if (t < best_ms) {

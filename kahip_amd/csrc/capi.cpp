@@ -435,6 +435,7 @@ int kahip_lp_cluster(int n, const int* xadj, const int* adjncy, const int* vwgt,
         eng.lp.prepare(LpMode::Cluster, g.g, n, subrounds, seed, bins, s);
         eng.lp.run(LpMode::Cluster, g.g, label.p, size.p, bound, rounds, seed, s, nullptr,
                    "lp_cluster");
+        eng.lp.check(s);
         if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
         const int nc = eng.cw.remap(g.g, label.p, map.p, s, nullptr);
         if (n) KH_CHECK(hipMemcpyAsync(cluster, map.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -505,6 +506,7 @@ int kahip_lp_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
         BinConfig bins;
         eng.lp.prepare(LpMode::Refine, g.g, k, subrounds, seed, bins, s);
         eng.lp.run(LpMode::Refine, g.g, dp.p, W.p, bound, rounds, seed, s, nullptr, "lp_refine");
+        eng.lp.check(s);
         if (moves_per_round) eng.lp.read_moves(rounds, moves_per_round, s);
         if (n) KH_CHECK(hipMemcpyAsync(part, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));

@@ -43,6 +43,35 @@ __host__ __device__ inline uint32_t hash3(uint32_t s, uint32_t a, uint32_t b) {
     return mix32(mix32(mix32(s ^ 0x85ebca6bU) ^ a) ^ b);
 }
 
+// Open-addressing tables of the kernels: keys >= 0, empty slot -1, cap a power of two, linear
+// probing from mix32(key). probe_claim returns the slot holding key, claiming the first empty
+// slot of its probe path when key is absent (fresh = true then), or -1 once all cap slots hold
+// other keys. Every table is sized so that -1 cannot occur; callers record it in a device error
+// flag that the host turns into an error, so no probe loop can run without end.
+template <class I>
+__device__ inline I probe_claim(int* K, I cap, int key, bool& fresh) {
+    I h = (I)(mix32((uint32_t)key) & (uint32_t)(cap - 1));
+    for (I step = 0; step < cap; ++step) {
+        const int old = atomicCAS(&K[h], -1, key);
+        if (old == -1 || old == key) {
+            fresh = old == -1;
+            return h;
+        }
+        h = (h + 1) & (cap - 1);
+    }
+    fresh = false;
+    return (I)-1;
+}
+template <class I>
+__device__ inline I probe_claim(int* K, I cap, int key) {
+    bool fresh;
+    return probe_claim(K, cap, key, fresh);
+}
+
+// Device error flags (one int per workspace, cleared before its kernels run, read by the host
+// after them).
+constexpr int kErrProbe = 1;  // a hash table probe found no slot (sizing invariant broken)
+
 constexpr uint32_t kSubSalt = 0x68bc21ebU;
 constexpr uint32_t kTieSalt = 0x2f6b7d93U;
 constexpr uint32_t kGolden = 0x9e3779b9U;

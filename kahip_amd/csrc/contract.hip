@@ -44,8 +44,18 @@ constexpr int kMidMaxDistinct = 1024;  // 24 KB of hash and 8 KB of sort keys (5
 constexpr int kGChunk = 4096;      // member edges per workgroup in the global tier
 constexpr int kGCap = 8192;        // LDS hash slots of a global-tier chunk
 constexpr int kGThreads = 512;
+static_assert(2 * kGChunk <= kGCap, "a global-tier chunk fills at most half its table");
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
+
+// The row kernels' early-exit flag lives in LDS and is polled between insertions without a
+// barrier: workgroup-scope atomic loads and stores keep the compiler from caching it.
+__device__ inline int lds_flag(const int* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lds_raise(int* f) {
+    __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 __global__ void leader_min(int n, const int* __restrict__ label, int* __restrict__ leader) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,9 +134,12 @@ __global__ __launch_bounds__(64) void rows_lds(
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
     const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
-    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count,
+    int* __restrict__ err) {
     constexpr int MAXD = CAP / 2;
     constexpr int kEpl = 4;
+    // with CHECK, inserts that have not yet seen `over` add at most 64 x kEpl entries past MAXD
+    static_assert(!CHECK || MAXD + 64 * kEpl <= CAP, "rows_lds table too small");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
     __shared__ int M[CAP];
@@ -199,20 +212,18 @@ __global__ __launch_bounds__(64) void rows_lds(
             for (int j = 0; j < kEpl; ++j) {
                 const int q = qs[j];
                 if (es[j] < 0 || q == p) continue;
-                if (CHECK && over) break;
-                uint32_t h = mix32((uint32_t)q) & (uint32_t)(cap - 1);
-                while (true) {
-                    const int old = atomicCAS(&K[h], -1, q);
-                    if (CHECK && old == -1) {
-                        if (atomicAdd(&used, 1) >= MAXD) over = 1;
-                    }
-                    if (old == -1 || old == q) {
-                        atomicAdd(&V[h], ws[j]);
-                        atomicMin(&M[h], es[j]);
-                        break;
-                    }
-                    h = (h + 1) & (uint32_t)(cap - 1);
+                if (CHECK && lds_flag(&over)) break;
+                // distinct entries: <= fdeg <= CAP / 2 without CHECK; with CHECK at most
+                // MAXD + 64 x kEpl before every lane has seen `over`
+                bool fresh;
+                const int h = probe_claim(K, cap, q, fresh);
+                if (h < 0) {
+                    *err = kErrProbe;
+                    continue;
                 }
+                if (CHECK && fresh && atomicAdd(&used, 1) >= MAXD) lds_raise(&over);
+                atomicAdd(&V[h], ws[j]);
+                atomicMin(&M[h], es[j]);
             }
         }
         __syncthreads();
@@ -273,10 +284,12 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
     const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
-    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count) {
+    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count,
+    int* __restrict__ err) {
     static_assert(CAP == 2 * MAXD && MAXD % THREADS == 0, "table shape");
     constexpr int kEpl = 4;
     constexpr int R = MAXD / THREADS;  // members per thread in the prefix scan
+    static_assert(MAXD + THREADS * kEpl <= CAP, "rows_lds_big table too small");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
     __shared__ int M[CAP];
@@ -338,7 +351,7 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
             }
             __syncthreads();
             for (int t0 = 0; t0 < total; t0 += THREADS * kEpl) {
-                if (over) break;  // no barrier in this loop
+                if (lds_flag(&over)) break;  // no barrier in this loop
                 int es[kEpl], qs[kEpl], ws[kEpl];
 #pragma unroll
                 for (int j = 0; j < kEpl; ++j) {
@@ -366,18 +379,17 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
                 for (int j = 0; j < kEpl; ++j) {
                     const int q = qs[j];
                     if (es[j] < 0 || q == p) continue;
-                    if (over) break;
-                    uint32_t h = mix32((uint32_t)q) & (uint32_t)(cap - 1);
-                    while (true) {
-                        const int old = atomicCAS(&K[h], -1, q);
-                        if (old == -1 && atomicAdd(&used, 1) >= MAXD) over = 1;
-                        if (old == -1 || old == q) {
-                            atomicAdd(&V[h], ws[j]);
-                            atomicMin(&M[h], es[j]);
-                            break;
-                        }
-                        h = (h + 1) & (uint32_t)(cap - 1);
+                    if (lds_flag(&over)) break;
+                    // at most MAXD distinct before `over`, plus THREADS x kEpl in flight: <= CAP
+                    bool fresh;
+                    const int h = probe_claim(K, cap, q, fresh);
+                    if (h < 0) {
+                        *err = kErrProbe;
+                        continue;
                     }
+                    if (fresh && atomicAdd(&used, 1) >= MAXD) lds_raise(&over);
+                    atomicAdd(&V[h], ws[j]);
+                    atomicMin(&M[h], es[j]);
                 }
             }
             __syncthreads();  // S is rewritten by the next batch; over is read below
@@ -464,7 +476,8 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     const int* __restrict__ xadj, const int* __restrict__ adjncy, const int* __restrict__ adjwgt,
     const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
     const int* __restrict__ mpre, const long long* __restrict__ offm_base, int* __restrict__ offm,
-    const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials) {
+    const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials,
+    int* __restrict__ err) {
     __shared__ int K[kGCap];
     __shared__ int V[kGCap];
     __shared__ int M[kGCap];
@@ -554,16 +567,13 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     for (int j = 0; j < PER; ++j) {
         const int q = qs[j];
         if (es[j] < 0 || q == p) continue;
-        uint32_t h = mix32((uint32_t)q) & (uint32_t)(kGCap - 1);
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, q);
-            if (old == -1 || old == q) {
-                atomicAdd(&V[h], ws[j]);
-                atomicMin(&M[h], es[j]);
-                break;
-            }
-            h = (h + 1) & (uint32_t)(kGCap - 1);
+        const int h = probe_claim(K, kGCap, q);  // <= kGChunk entries in kGCap slots
+        if (h < 0) {
+            *err = kErrProbe;
+            continue;
         }
+        atomicAdd(&V[h], ws[j]);
+        atomicMin(&M[h], es[j]);
     }
     for (int i = threadIdx.x; i <= P; i += kGThreads) bcur[i] = 0;
     __syncthreads();
@@ -639,7 +649,7 @@ __global__ __launch_bounds__(512) void gmerge(
     int n_tasks, const long long* __restrict__ ct_base, const int* __restrict__ posT,
     const int4* __restrict__ bucketed, const int* __restrict__ obase, int* __restrict__ rowcnt,
     unsigned* __restrict__ okey, unsigned long long* __restrict__ oval,
-    int* __restrict__ big_tasks, int* __restrict__ n_big) {
+    int* __restrict__ big_tasks, int* __restrict__ n_big, int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
@@ -671,16 +681,13 @@ __global__ __launch_bounds__(512) void gmerge(
     __syncthreads();
     for (int i = r0 + threadIdx.x; i < r1; i += 512) {
         const int4 e = bucketed[i];
-        uint32_t h = mix32((uint32_t)e.x) & (uint32_t)(cap - 1);
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, e.x);
-            if (old == -1 || old == e.x) {
-                atomicAdd(&V[h], e.y);
-                atomicMin(&M[h], e.z);
-                break;
-            }
-            h = (h + 1) & (uint32_t)(cap - 1);
+        const int h = probe_claim(K, cap, e.x);  // cap >= 2 x the range's entries
+        if (h < 0) {
+            *err = kErrProbe;
+            continue;
         }
+        atomicAdd(&V[h], e.y);
+        atomicMin(&M[h], e.z);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < cap; i += 512)
@@ -706,7 +713,8 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     const int* __restrict__ posT, const int4* __restrict__ bucketed,
     const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
     int* __restrict__ TV, int* __restrict__ TM, const int* __restrict__ obase,
-    int* __restrict__ rowcnt, unsigned* __restrict__ okey, unsigned long long* __restrict__ oval) {
+    int* __restrict__ rowcnt, unsigned* __restrict__ okey, unsigned long long* __restrict__ oval,
+    int* __restrict__ err) {
     __shared__ int row, nout, base;
     const int t = blockIdx.x;
     if (t >= n_big) return;
@@ -727,16 +735,13 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     int* Mn = TM + toff[t];
     for (int i = r0 + threadIdx.x; i < r1; i += 1024) {
         const int4 e = bucketed[i];
-        long long h = (long long)(mix32((uint32_t)e.x) & (uint32_t)(cap - 1));
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, e.x);
-            if (old == -1 || old == e.x) {
-                atomicAdd(&V[h], e.y);
-                atomicMin(&Mn[h], e.z);
-                break;
-            }
-            h = (h + 1) & (cap - 1);
+        const long long h = probe_claim(K, cap, e.x);  // cap >= 2 x the range's entries
+        if (h < 0) {
+            *err = kErrProbe;
+            continue;
         }
+        atomicAdd(&V[h], e.y);
+        atomicMin(&Mn[h], e.z);
     }
     __threadfence();  // the table's atomics complete before it is read back (rare path)
     __syncthreads();
@@ -900,6 +905,8 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     ovl2_list_.reserve(nc);
     ovl3_list_.reserve(nc);
     fill_async(ovl_count_.p, 3, 0, s);
+    err_.reserve(1);
+    fill_async(err_.p, 1, 0, s);
     if (nc > 0) {
         // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
@@ -918,12 +925,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             hipLaunchKernelGGL((rows_lds<2 * kRowSmall, false>), dim3(rc[0]), dim3(64), 0, s,
                                rsmall_.p, rc[0], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
                                members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
-                               ovl_count_.p);
+                               ovl_count_.p, err_.p);
         if (rc[1] > 0)
             hipLaunchKernelGGL((rows_lds<kRowCap, true>), dim3(rc[1]), dim3(64), 0, s, rmid_.p,
                                rc[1], g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
                                fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
-                               ovl_count_.p);
+                               ovl_count_.p, err_.p);
     }
     if (nc > 0) {
         // overflow of rows_lds -> mid tier (<= kMidMaxDistinct) -> big tier (<= kBigMaxDistinct)
@@ -932,11 +939,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         hipLaunchKernelGGL((rows_lds_big<kMidThreads, 2 * kMidMaxDistinct, kMidMaxDistinct>),
                            dim3(256 * 5), dim3(kMidThreads), 0, s, ovl_list_.p, ovl_count_.p,
                            g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p,
-                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl3_list_.p, ovl_count_.p + 2);
+                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl3_list_.p, ovl_count_.p + 2,
+                           err_.p);
         hipLaunchKernelGGL((rows_lds_big<kBigThreads, kBigCap, kBigMaxDistinct>), dim3(256),
                            dim3(kBigThreads), 0, s, ovl3_list_.p, ovl_count_.p + 2, g.xadj,
                            g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p, tpos_.p,
-                           cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1);
+                           cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1, err_.p);
     }
     int novl = 0;
     KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1016,13 +1024,14 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
         hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
                            gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
-                           mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p, gpart_.p);
+                           mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p, gpart_.p,
+                           err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
         hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,
                            goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
         hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(512), 0, s, gchunk_prefix_.p,
                            gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p, obase_.p,
-                           growcnt_.p, okey_.p, oval_.p, gbig_.p, gnbig_.p);
+                           growcnt_.p, okey_.p, oval_.p, gbig_.p, gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -1061,7 +1070,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
                                gchunk_prefix_.p, gtask_prefix_.p, novl, gct_base_.p, gpos_.p,
                                gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, obase_.p,
-                               growcnt_.p, okey_.p, oval_.p);
+                               growcnt_.p, okey_.p, oval_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
         }
         hipLaunchKernelGGL(row_counts_to_cdeg, dim3(grid_for(novl, 256)), dim3(256), 0, s,
@@ -1081,9 +1090,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
     }
     scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
-    int mc = 0;
+    int mc = 0, err = 0;
     KH_CHECK(hipMemcpyAsync(&mc, out.xadj_dev() + nc, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipMemcpyAsync(&err, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
+    KH_REQUIRE(err == 0, "contraction: a hash table probe found no free slot (kernel table sizing "
+                         "invariant violated); the result is invalid");
     out.alloc_edges(mc);
     if (nc > 0) {
         ProfScope ps(prof, "contract_compact", 16.0 * mc, 0, s);

@@ -47,6 +47,7 @@ private:
     DevBuf<int4> gpart_, gbuck_;
     DevBuf<unsigned> okey_, okey2_;
     DevBuf<unsigned long long> oval_, oval2_;
+    DevBuf<int> err_;  // device error flag (kErrProbe) of the last contract()
 };
 
 // fine_part[v] = coarse_part[map[v]] (reference: graph_hierarchy.cpp:54-58).

@@ -233,6 +233,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, cseed, p.bins, s);
         lp.run(LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
                "lp_cluster");
+        lp.check(s);
         long long edges_round = 0;
         for (int i = 0; i < p.subrounds * kNumBins; ++i) edges_round += lp.host_edges[i];
         S.edges_cluster += edges_round * p.cluster_rounds;
@@ -245,6 +246,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, eseed, p.bins, s);
             lp.run(LpMode::Cluster, rated, elabel.p, size.p, eb, p.cluster_rounds, eseed, s, prof,
                    "lp_cluster");
+            lp.check(s);
             S.edges_cluster += edges_round * p.cluster_rounds;
             overlay(fine.n, label.p, elabel.p, s);
         }
@@ -316,6 +318,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         else
             lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, nullptr,
                    "lp_refine");
+        lp.check(s);
         S.edges_refine += lp.traversed(s);
     }
     KH_CHECK(hipEventRecord(ev_refine.b, s));

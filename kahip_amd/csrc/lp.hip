@@ -298,16 +298,13 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     for (int i = 0; i < IT; ++i) {
         if (r[i].label < 0) continue;
         const int tag = (r[i].label << 1) | (r[i].departure ? 1 : 0);
-        uint32_t h = mix32((uint32_t)tag) & (uint32_t)(HC - 1);
-        while (true) {
-            const int old = atomicCAS(&h_tag[h], -1, tag);
-            if (old == -1 || old == tag) {
-                atomicAdd(&h_w[h], r[i].w);
-                atomicMax(&h_key[h], r[i].key);
-                break;
-            }
-            h = (h + 1) & (uint32_t)(HC - 1);
+        const int h = probe_claim(h_tag, HC, tag);  // at most IT x NB tags in HC slots
+        if (h < 0) {
+            *a.err = kErrProbe;
+            continue;
         }
+        atomicAdd(&h_w[h], r[i].w);
+        atomicMax(&h_key[h], r[i].key);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < HC; i += kBlock) {
@@ -388,15 +385,12 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
             atomicAdd(&V[lab], w);
             continue;
         }
-        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(cap - 1);
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, lab);
-            if (old == -1 || old == lab) {
-                atomicAdd(&V[h], w);
-                break;
-            }
-            h = (h + 1) & (uint32_t)(cap - 1);
+        const int h = probe_claim(K, cap, lab);  // cap >= 2 x degree
+        if (h < 0) {
+            *a.err = kErrProbe;
+            continue;
         }
+        atomicAdd(&V[h], w);
     }
     __syncthreads();
     const int nw = valid ? a.vwgt[v] : 0;
@@ -616,16 +610,12 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
     for (int j = 0; j < EPT; ++j) {
         const int lab = labs[j];
         if (lab < 0) continue;
-        const int w = ws[j];
-        uint32_t h = mix32((uint32_t)lab) & (uint32_t)(cap - 1);
-        while (true) {
-            const int old = atomicCAS(&K[h], -1, lab);
-            if (old == -1 || old == lab) {
-                atomicAdd(&V[h], w);
-                break;
-            }
-            h = (h + 1) & (uint32_t)(cap - 1);
+        const int h = probe_claim(K, cap, lab);  // cap >= 2 x the chunk's edges
+        if (h < 0) {
+            *a.err = kErrProbe;
+            continue;
         }
+        atomicAdd(&V[h], ws[j]);
     }
     __syncthreads();
     if constexpr (MULTI) if (nchunks > 1) {
@@ -761,6 +751,7 @@ void hub_bucket_merge(
     constexpr int THREADS = LEVEL == 0 ? kTinyThreads : (LEVEL == 1 ? kSmallThreads : kFinThreads);
     constexpr int CAP = LEVEL == 0 ? kTinyCap : (LEVEL == 1 ? kSmallCap : kFinCap);
     constexpr int MAXE = LEVEL == 0 ? kTinyMax : kSmallMax;  // queued beyond (levels 0, 1)
+    static_assert(2 * MAXE <= CAP && 2 * kFinLds <= kFinCap, "merge tables fill at most half");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
     const int n_iter = LEVEL == 0 ? n_tasks : *in_n;
@@ -807,15 +798,13 @@ void hub_bucket_merge(
         __syncthreads();
         for (int i = r0 + threadIdx.x; i < r1; i += THREADS) {
             const int2 kv = bucketed[i];
-            long long h = (long long)(mix32((uint32_t)kv.x) & (uint32_t)(cap - 1));
-            while (true) {
-                const int old = atomicCAS(&TK[h], -1, kv.x);
-                if (old == -1 || old == kv.x) {
-                    atomicAdd(&TV[h], kv.y);
-                    break;
-                }
-                h = (h + 1) & (cap - 1);
+            // LDS: cap >= 2 x the bucket's entries; global: cap >= 2 x the hub's degree
+            const long long h = probe_claim(TK, cap, kv.x);
+            if (h < 0) {
+                *a.err = kErrProbe;
+                continue;
             }
+            atomicAdd(&TV[h], kv.y);
         }
         // Only the global table needs the device-scope fence (it waits for the atomics and
         // invalidates the vector L1); it is costly, so the LDS path, the common one, skips it.
@@ -1272,6 +1261,7 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     n_mid_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     trav_.reserve(64);
+    err_.reserve(1);
 
     fill_async(counts_.p, L + kMaxSub, 0, s);
     fill_async(cursor_.p, L + kMaxSub, 0, s);
@@ -1556,6 +1546,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     fill_async(n_mid_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
     fill_async(trav_.p, 64, 0ull, s);
+    fill_async(err_.p, 1, 0, s);
     if (!cl) {
         fill_async(active_.p, g.n, (uint8_t)1, s);
         fill_async(next_active_.p, g.n, (uint8_t)0, s);
@@ -1578,6 +1569,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.prop_ok = prop_ok_.p;
     a.trav = trav_.p;
     a.big = big_.p;
+    a.err = err_.p;
     // Refinement hubs of more than one chunk keep per-block connection counters across the
     // rounds: one scan builds them, every applied move updates its hub neighbours' counters, and
     // the hubs then decide from the counters instead of rescanning their edges each round.
@@ -1769,6 +1761,14 @@ void LpWorkspace::read_moves(int rounds, int* out, hipStream_t s) {
     if (rounds <= 0) return;
     KH_CHECK(hipMemcpyAsync(out, moves_.p, sizeof(int) * rounds, hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
+}
+
+void LpWorkspace::check(hipStream_t s) {
+    int e = 0;
+    KH_CHECK(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipStreamSynchronize(s));
+    KH_REQUIRE(e == 0, "label propagation: a hash table probe found no free slot (kernel table "
+                       "sizing invariant violated); the result is invalid");
 }
 
 long long LpWorkspace::traversed(hipStream_t s) {

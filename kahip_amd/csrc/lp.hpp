@@ -39,6 +39,7 @@ struct LpArgs {
     const int* hub_slot;
     int* hub_cnt;
     int* big_from;
+    int* err;  // device error flag (kErrProbe), cleared by run(), read by check()
 };
 
 struct LpStats {
@@ -64,6 +65,8 @@ public:
     void read_moves(int rounds, int* out, hipStream_t s);
     // Directed edges traversed by the last refinement run (synchronises the stream).
     long long traversed(hipStream_t s);
+    // Throws if a kernel of the last run() set the device error flag (synchronises the stream).
+    void check(hipStream_t s);
 
     int host_counts[kMaxSub * kNumBins] = {};
     int host_wave_small_[kMaxSub] = {};  // clustering: wave-list nodes of the small-LDS kernel
@@ -116,6 +119,7 @@ private:
     DevBuf<int> hub_key_, hub_val_;
     DevBuf<int> dense_off_, dense_cnt_;  // chunked dense refinement hubs: per-hub counters
     DevBuf<int> hub_slot_, big_from_;
+    DevBuf<int> err_;
     long long dense_total_ = 0;
     bool dense_all_ = false;
     long long hub_total_ = 0;

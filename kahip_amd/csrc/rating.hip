@@ -512,6 +512,7 @@ void RatingWorkspace::sweeps_and_sketch(const DGraph& g, int bound, uint32_t see
             LpWorkspace::init_cluster_labels(g, lab, size_.p, s);
             lp.run(LpMode::Cluster, g, lab, size_.p, bound, 1, sd, s, nullptr, "rating_sweep");
         }
+        lp.check(s);
     }
     hipLaunchKernelGGL(node_rec_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.xadj,
                        g.vwgt, sig_.p, lab_.p, lab_.p + g.n, lab_.p + 2LL * g.n,
