@@ -45,10 +45,6 @@ WORKLOADS = {
 
 
 BATCH = dict(graphs=4096, log2_min=10.0, log2_max=16.0, k=8, seed=1, micro=64)
-# The batch's LP round loops can run as captured hipGraphs (use_graph=True, same partition);
-# direct launches measured faster on this workload (whole batch 863 vs 874 GPU ms, micro-batches
-# of 64 graphs 45 vs 55 GPU ms): each call captures and instantiates a new graph.
-USE_GRAPH = False
 
 
 def run_batch(K, args):
@@ -65,11 +61,11 @@ def run_batch(K, args):
     bins, tune_ms = K.autotune_bins(b, k=w["k"])
     log(f"autotuned bins {bins} (ms per candidate {[round(t, 2) for t in tune_ms]})")
     for i in range(args.warmup):
-        b.partition(k=w["k"], seed=0, bins=bins, use_graph=USE_GRAPH)
+        b.partition(k=w["k"], seed=0, bins=bins)
     walls, stats = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        _, cuts, mbw, st = b.partition(k=w["k"], seed=0, bins=bins, use_graph=USE_GRAPH)
+        _, cuts, mbw, st = b.partition(k=w["k"], seed=0, bins=bins)
         walls.append(time.perf_counter() - t)
         stats.append(st)
         log(f"step {i}: wall {walls[-1]*1e3:.1f} ms gpu {st['ms_coarsen'] + st['ms_refine'] + st['ms_cut']:.1f} ms "
@@ -86,7 +82,7 @@ def run_batch(K, args):
             gs.append((x[off[g]:off[g + 1] + 1] - e0, a[e0:e1] - off[g], None, ew[e0:e1]))
         mb = K.Batch.from_graphs(gs)
         t = time.perf_counter()
-        mb.partition(k=w["k"], seed=0, bins=bins, use_graph=USE_GRAPH)
+        mb.partition(k=w["k"], seed=0, bins=bins)
         lat += [time.perf_counter() - t] * (hi - lo)
         mb.free()
     b.free()

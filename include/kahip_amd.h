@@ -40,9 +40,9 @@ typedef struct {
     int ip_tries;        /* BFS-grown starts per bisection of the initial partitioner (32) */
     int fm_rounds;       /* > 0: CPU 2-way FM over adjacent block pairs on the finest level after
                             the LP refinement, at most this many passes (0 = off) */
-    int graph_refine;    /* 1: run each level's LP refinement rounds as one captured hipGraph;
-                            0 (default): direct launches, measured faster on every tested size;
-                            the partition does not depend on it */
+    int graph_refine;    /* must be 0 (the default): hipGraph-captured refinement rounds are
+                            disabled after an unexplained host fault in that path under a
+                            profiler; non-zero is rejected with an error */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);
@@ -202,7 +202,7 @@ int kahip_batch_info(kahip_batch_t b, int* num_graphs, int* n, long long* m2, in
 int kahip_batch_to_host(kahip_batch_t b, int* xadj, int* adjncy, int* vwgt, int* adjwgt);
 void kahip_batch_free(kahip_batch_t b);
 /* part: n local block ids (may be NULL); cuts / max_block_weights: G values (may be NULL).
- * use_graph != 0 runs every LP round loop as one captured hipGraph (same results). */
+ * use_graph must be 0: hipGraph-captured round loops are disabled (non-zero is an error). */
 int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_graph, int* part,
                           long long* cuts, long long* max_block_weights,
                           kahip_batch_stats* stats);

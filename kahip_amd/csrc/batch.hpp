@@ -27,8 +27,8 @@ struct BatchStats {
 };
 
 // g: the union (level 0), node_off: host array of G + 1 node offsets (every graph has >= 1
-// node). part_out: host array of g.n local block ids (0..k-1 per graph). With use_graph, the LP
-// round loops run as captured hipGraphs.
+// node). part_out: host array of g.n local block ids (0..k-1 per graph). use_graph must be false
+// (the captured round loops are disabled; an error otherwise).
 void partition_batch(Engine& eng, const DGraph& g, int G, const int* node_off,
                      const PartitionParams& p, int* part_out, BatchStats* st,
                      bool use_graph = false);

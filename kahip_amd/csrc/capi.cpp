@@ -75,7 +75,9 @@ PartitionParams to_params(const kahip_params* q) {
     KH_REQUIRE(p.ip_tries >= 1, "ip_tries must be >= 1");
     p.fm_rounds = q->fm_rounds;
     KH_REQUIRE(p.fm_rounds >= 0, "fm_rounds must be >= 0");
-    p.graph_refine = q->graph_refine;
+    KH_REQUIRE(q->graph_refine == 0, "hipGraph-captured LP rounds are disabled: the captured path hit a host segmentation "
+               "fault under rocprofv3 whose cause was not found");
+    p.graph_refine = 0;
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
     for (int b = 0; b + 1 < kNumBins; ++b)

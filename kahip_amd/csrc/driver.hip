@@ -312,12 +312,8 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         const uint32_t rseed = refine_seed(p.seed, l);
         lp.prepare(LpMode::Refine, gl, p.k, p.subrounds, rseed, p.bins, s);
         // the profiler times the clustering and contraction kernels, not the refinement rounds
-        if (p.graph_refine > 0)
-            run_lp_graph(lp, LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s,
-                         "lp_refine");
-        else
-            lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, nullptr,
-                   "lp_refine");
+        lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, nullptr,
+               "lp_refine");
         lp.check(s);
         S.edges_refine += lp.traversed(s);
     }

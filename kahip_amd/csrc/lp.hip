@@ -1540,7 +1540,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     const int S = subrounds_;
     const bool cl = mode == LpMode::Cluster;
     KH_REQUIRE(cl || !(dense_wave_ || dense_hub_) || num_labels_ > 0, "prepare() not called");
-    // no allocation here: run() may be captured into a hipGraph (prepare() sized everything)
+    // no allocation here: prepare() sized everything
     fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_big_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_mid_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
@@ -1734,27 +1734,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         }
     }
     KH_CHECK(hipGetLastError());
-}
-
-void run_lp_graph(LpWorkspace& lp, LpMode mode, const DGraph& g, int* label, int* size,
-                  int bound, int rounds, uint32_t seed, hipStream_t s, const char* phase,
-                  const int* lbound) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    KH_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-        lp.run(mode, g, label, size, bound, rounds, seed, s, nullptr, phase, lbound);
-    } catch (...) {
-        (void)hipStreamEndCapture(s, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        throw;
-    }
-    KH_CHECK(hipStreamEndCapture(s, &graph));
-    KH_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    KH_CHECK(hipGraphLaunch(exec, s));
-    KH_CHECK(hipStreamSynchronize(s));
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
 }
 
 void LpWorkspace::read_moves(int rounds, int* out, hipStream_t s) {

@@ -130,12 +130,6 @@ void fill_async(unsigned long long* p, long long n, unsigned long long value, hi
 void fill_async(uint8_t* p, long long n, uint8_t value, hipStream_t s);
 void fill_async(long long* p, long long n, long long value, hipStream_t s);
 
-// lp.run(...) without a profiler, captured as one hipGraph and launched (the same launches; run()
-// allocates nothing and clears with fill kernels, so it can be captured). Synchronises s.
-void run_lp_graph(LpWorkspace& lp, LpMode mode, const DGraph& g, int* label, int* size,
-                  int bound, int rounds, uint32_t seed, hipStream_t s, const char* phase,
-                  const int* lbound = nullptr);
-
 // W[b] = sum of vwgt over nodes in block b (k <= 65536).
 void compute_block_weights(const DGraph& g, const int* part, int k, int* W, hipStream_t s);
 

@@ -53,6 +53,8 @@ def test_batch_matches_mirror(k, seed, stop):
     b.free()
 
 
+@pytest.mark.skip(reason="hipGraph-captured LP rounds are disabled: that path hit a host "
+                         "segmentation fault under rocprofv3 whose cause was not found")
 def test_batch_hipgraph_equals_eager():
     gs = small_batch()
     b = K.Batch.from_graphs(gs)

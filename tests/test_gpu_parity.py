@@ -313,6 +313,12 @@ def test_pipeline_fm_matches_mirror(G, name, k, seed):
     assert O.block_weights(vw, k, got).max() <= info["U"]
 
 
+def test_graph_capture_options_are_rejected(G):
+    x, a, vw, ew = G["manual"]
+    with pytest.raises(K.KahipError, match="disabled"):
+        K.partition(x, a, vw, ew, k=2, graph_refine=1)
+
+
 def test_pipeline_deterministic(G):
     x, a, vw, ew = G["rmat12"]
     p1, s1 = K.partition(x, a, vw, ew, k=16, seed=5)
@@ -320,6 +326,11 @@ def test_pipeline_deterministic(G):
     assert np.array_equal(p1, p2) and s1["cut"] == s2["cut"]
 
 
+GRAPH_OFF = ("hipGraph-captured LP rounds are disabled: that path hit a host segmentation fault "
+             "under rocprofv3 whose cause was not found")
+
+
+@pytest.mark.skip(reason=GRAPH_OFF)
 @pytest.mark.parametrize("name,k,seed", [("rgg15", 8, 1), ("rmat12", 16, 3), ("star1500", 4, 0)])
 def test_graph_captured_refinement_matches_mirror(G, name, k, seed):
     """graph_refine = 1 replays each level's refinement rounds from a captured hipGraph; the
