@@ -38,8 +38,8 @@ typedef struct {
     int ensemble;        /* LP clusterings per level, overlaid (1 = single clustering; the
                             STRONGSOCIAL preset uses 18/17/15/7/3 for k <= 3/7/15/31/more) */
     int ip_tries;        /* BFS-grown starts per bisection of the initial partitioner (32) */
-    int fm_rounds;       /* > 0: CPU 2-way FM over adjacent block pairs on the finest level after
-                            the LP refinement, at most this many passes (0 = off) */
+    int fm_rounds;       /* > 0: CPU 2-way FM over adjacent block pairs on every level after its
+                            LP refinement, at most this many passes per level (0 = off) */
     int graph_refine;    /* must be 0 (the default): hipGraph-captured refinement rounds are
                             disabled after an unexplained host fault in that path under a
                             profiler; non-zero is rejected with an error */
@@ -74,7 +74,7 @@ typedef struct {
     long long max_block_weight;
     double ms_rating; /* rating time, part of ms_coarsen */
     int reordered;    /* the input was renumbered by degree */
-    double ms_fm;     /* CPU pairwise FM on the finest level (kahip_params.fm_rounds > 0) */
+    double ms_fm;     /* CPU pairwise FM on every level (kahip_params.fm_rounds > 0) */
     long long fm_gain; /* its cut reduction */
 } kahip_stats;
 
@@ -132,6 +132,15 @@ int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_pa
 int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
                             const int* adjwgt, int k, long long block_bound, int repetitions,
                             unsigned long long seed, int* part);
+
+/* CPU post-pass of the perfectly balanced modes (the reference runs cycle_refinement there,
+ * interface/kaHIP_interface.cpp:123-132): moves nodes out of blocks heavier than `bound`, best
+ * cut change first with gains kept current, until every block is within the bound or no move
+ * is possible, then refines by 2-way FM over adjacent block pairs under the same bound. part
+ * (in/out) in [0, k); *cut (nullable) receives the resulting cut. No GPU. */
+int kahip_rebalance(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, long long bound, unsigned long long seed,
+                    int* part, long long* cut);
 
 /* CPU refinement of a k-way partition by 2-way FM on the pairs of adjacent blocks (replaces
  * the quotient-graph refinement of lib/partition/uncoarsening/refinement/quotient_graph_refinement/

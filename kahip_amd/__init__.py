@@ -103,7 +103,7 @@ EXPORTS = [
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
     "kahip_mlp_train", "kahip_batch_from_host", "kahip_batch_generate", "kahip_batch_info",
     "kahip_batch_to_host", "kahip_batch_free", "kahip_batch_partition", "kahip_autotune_bins",
-    "kahip_fm_refine",
+    "kahip_fm_refine", "kahip_rebalance",
 ]
 
 _lib = None
@@ -160,6 +160,8 @@ def lib():
     L.kahip_project.argtypes = [C.c_int, _i32p, C.c_int, _i32p, _i32p]
     L.kahip_initial_partition.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
                                           C.c_int, C.c_ulonglong, _i32p]
+    L.kahip_rebalance.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
+                                  C.c_ulonglong, _i32p, C.POINTER(C.c_longlong)]
     L.kahip_fm_refine.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong, C.c_int,
                                   C.c_ulonglong, _i32p, C.POINTER(C.c_longlong)]
     L.kahip_edge_features.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_uint, _f32p]
@@ -443,6 +445,18 @@ def fm_refine(xadj, adjncy, vwgt, adjwgt, k, block_bound, part, rounds=3, seed=0
                                  _p(vwgt), _p(adjwgt), int(k), int(block_bound), int(rounds),
                                  int(seed) & 0xFFFFFFFFFFFFFFFF, out, C.byref(gain)))
     return out, int(gain.value)
+
+
+def rebalance(xadj, adjncy, vwgt, adjwgt, k, bound, part, seed=0):
+    """CPU post-pass of the perfectly balanced modes (rebalancing to `bound`, then 2-way FM under
+    it); returns (part, cut). No GPU."""
+    xadj, adjncy = _i32(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    p = np.array(part, dtype=np.int32) if n else np.zeros(1, np.int32)
+    cut = C.c_longlong(0)
+    _check(lib().kahip_rebalance(n, xadj, adjncy if len(adjncy) else np.zeros(1, np.int32),
+                                 _p(vwgt), _p(adjwgt), k, int(bound), seed, p, C.byref(cut)))
+    return p[:n], int(cut.value)
 
 
 def fm_seed(seed):

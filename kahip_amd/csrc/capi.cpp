@@ -4,8 +4,10 @@
 // last profile summary.
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <queue>
 #include <sstream>
@@ -75,8 +77,9 @@ PartitionParams to_params(const kahip_params* q) {
     KH_REQUIRE(p.ip_tries >= 1, "ip_tries must be >= 1");
     p.fm_rounds = q->fm_rounds;
     KH_REQUIRE(p.fm_rounds >= 0, "fm_rounds must be >= 0");
-    KH_REQUIRE(q->graph_refine == 0, "hipGraph-captured LP rounds are disabled: the captured path hit a host segmentation "
-               "fault under rocprofv3 whose cause was not found");
+    KH_REQUIRE(q->graph_refine == 0,
+               "hipGraph-captured LP rounds are disabled: the captured path hit a host "
+               "segmentation fault under rocprofv3 whose cause was not found");
     p.graph_refine = 0;
     KH_REQUIRE(p.rating != RatingMode::Learned || p.mlp, "learned rating needs mlp_weights");
     KH_REQUIRE(p.bins.hi[4] <= 512, "bins[4] must be <= 512 (LDS hash capacity)");
@@ -144,10 +147,17 @@ void fill_stats(const PartitionStats& s, kahip_stats* o) {
     o->fm_gain = s.fm_gain;
 }
 
-// Greedy CPU rebalancing for kaffpa_balance(perfectly_balance): while a block exceeds `bound`,
-// move the node of an overloaded block whose move to a block with room loses the least cut
-// (neighbouring blocks first, otherwise the lightest block). Stands in for the reference's
-// cycle_refinement post-pass (interface/kaHIP_interface.cpp:123-132), which is out of scope.
+// CPU rebalancing to a block bound, the post-pass of the perfectly balanced modes. The reference
+// runs cycle_refinement there (interface/kaHIP_interface.cpp:123-132, out of this path's scope);
+// this stand-in moves single nodes out of overloaded blocks in order of cut change, with the
+// gains kept current: a max-heap holds, for nodes of overloaded blocks, (gain, node, target) with
+// target the adjacent block of highest connection that has room (else the lightest block with
+// room) and gain = conn(target) - conn(own); a popped entry whose node moved, whose block is no
+// longer overloaded or whose target filled up is recomputed and pushed again. After a move the
+// moved node's neighbours in overloaded blocks are re-evaluated, so the moves grow from the
+// boundary inwards instead of cutting isolated nodes out. Ties: higher gain, then lower node
+// id, then lower target. Stops when no block exceeds the bound or no node of an overloaded block
+// has a target with room. Returns the cut of the result.
 long long rebalance(int n, const int* xadj, const int* adjncy, const int* vwgt,
                     const int* adjwgt, int k, long long bound, int* part) {
     std::vector<long long> W(k, 0);
@@ -156,47 +166,89 @@ long long rebalance(int n, const int* xadj, const int* adjncy, const int* vwgt,
     for (int v = 0; v < n; ++v) W[part[v]] += vw(v);
     std::vector<long long> conn(k, 0);
     std::vector<int> touched;
-    for (int guard = 0; guard < 4 * n + 16; ++guard) {
-        int over = -1;
-        for (int b = 0; b < k; ++b)
-            if (W[b] > bound && (over < 0 || W[b] > W[over])) over = b;
-        if (over < 0) break;
-        int lightest = (int)(std::min_element(W.begin(), W.end()) - W.begin());
-        long long best_loss = 0;
-        int best_v = -1, best_t = -1;
-        for (int v = 0; v < n; ++v) {
-            if (part[v] != over) continue;
-            for (long long e = xadj[v]; e < xadj[v + 1]; ++e) {
-                const int b = part[adjncy[e]];
-                if (conn[b] == 0) touched.push_back(b);
-                conn[b] += ew(e);
+    // lightest block, kept with a lazy min-heap of (weight, block)
+    using WB = std::pair<long long, int>;
+    std::priority_queue<WB, std::vector<WB>, std::greater<WB>> light;
+    for (int b = 0; b < k; ++b) light.push({W[b], b});
+    auto lightest = [&]() {
+        while (light.top().first != W[light.top().second]) light.pop();
+        return light.top().second;
+    };
+    // best (gain, target) of v, or target -1
+    auto best_move = [&](int v, long long& gain) -> int {
+        const int own = part[v];
+        for (long long e = xadj[v]; e < xadj[v + 1]; ++e) {
+            const int b = part[adjncy[e]];
+            if (conn[b] == 0) touched.push_back(b);
+            conn[b] += ew(e);
+        }
+        int t = -1;
+        long long tc = -1;
+        for (int b : touched)
+            if (b != own && W[b] + vw(v) <= bound && (conn[b] > tc || (conn[b] == tc && b < t))) {
+                t = b;
+                tc = conn[b];
             }
-            const long long own = conn[over];
-            int t = -1;
-            long long tc = -1;
-            for (int b : touched)
-                if (b != over && W[b] + vw(v) <= bound && conn[b] > tc) {
-                    t = b;
-                    tc = conn[b];
-                }
-            if (t < 0 && W[lightest] + vw(v) <= bound && lightest != over) {
-                t = lightest;
-                tc = conn[lightest];
-            }
-            for (int b : touched) conn[b] = 0;
-            touched.clear();
-            if (t < 0) continue;
-            const long long loss = own - tc;
-            if (best_v < 0 || loss < best_loss) {
-                best_loss = loss;
-                best_v = v;
-                best_t = t;
+        if (t < 0) {
+            const int l = lightest();
+            if (l != own && W[l] + vw(v) <= bound) {
+                t = l;
+                tc = conn[l];
             }
         }
-        if (best_v < 0) break;
-        W[over] -= vw(best_v);
-        W[best_t] += vw(best_v);
-        part[best_v] = best_t;
+        gain = t >= 0 ? tc - conn[own] : 0;
+        for (int b : touched) conn[b] = 0;
+        touched.clear();
+        return t;
+    };
+    struct Entry {
+        long long gain;
+        int v, t;
+        bool operator<(const Entry& o) const {  // max-heap order
+            if (gain != o.gain) return gain < o.gain;
+            if (v != o.v) return v > o.v;
+            return t > o.t;
+        }
+    };
+    std::priority_queue<Entry> heap;
+    std::vector<long long> stamp(n, -1);  // gain of v's current heap entry (-1: none valid)
+    std::vector<int> stamp_t(n, -1);
+    auto consider = [&](int v) {
+        if (W[part[v]] <= bound) return;
+        long long g = 0;
+        const int t = best_move(v, g);
+        if (t < 0) return;
+        if (stamp_t[v] == t && stamp[v] == g) return;  // the same entry is already queued
+        stamp[v] = g;
+        stamp_t[v] = t;
+        heap.push(Entry{g, v, t});
+    };
+    for (int v = 0; v < n; ++v) consider(v);
+    while (!heap.empty()) {
+        const Entry top = heap.top();
+        heap.pop();
+        const int v = top.v;
+        if (stamp_t[v] != top.t || stamp[v] != top.gain) continue;  // superseded
+        stamp_t[v] = -1;
+        const int own = part[v];
+        if (W[own] <= bound) continue;
+        long long g = 0;
+        const int t = best_move(v, g);
+        if (t < 0) continue;
+        if (t != top.t || g != top.gain) {  // stale: queue the current move instead
+            stamp[v] = g;
+            stamp_t[v] = t;
+            heap.push(Entry{g, v, t});
+            continue;
+        }
+        W[own] -= vw(v);
+        W[t] += vw(v);
+        light.push({W[own], own});
+        light.push({W[t], t});
+        part[v] = t;
+        for (long long e = xadj[v]; e < xadj[v + 1]; ++e) consider(adjncy[e]);
+        if (W[t] + 1 > bound)  // the target filled up: queued moves into it are recomputed
+            continue;
     }
     long long cut = 0;
     for (int v = 0; v < n; ++v)
@@ -224,6 +276,73 @@ int run_kaffpa(int n, const int* vwgt, const int* xadj, const int* adjcwgt, cons
 void report(const char* fn, int* edgecut) {
     std::fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
     if (edgecut) *edgecut = -1;
+}
+
+// The perfectly balanced post-pass: rebalance() to `bound`, then 2-way FM over the adjacent
+// block pairs under the same bound, allowed to exceed it by one node weight within a pass so
+// that balanced pairs can exchange nodes (only prefixes as balanced as the pass start are kept).
+// Returns the cut.
+long long balance_postpass(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                           const int* adjwgt, int k, long long bound, uint64_t seed, int* part) {
+    long long cut = rebalance(n, xadj, adjncy, vwgt, adjwgt, k, bound, part);
+    if (n == 0 || k < 2) return cut;
+    HostGraph g;
+    g.n = n;
+    g.xadj.assign(xadj, xadj + n + 1);
+    g.adjncy.assign(adjncy, adjncy + xadj[n]);
+    g.adjwgt.resize(xadj[n]);
+    g.vwgt.resize(n);
+    long long wmax = 1;
+    for (int e = 0; e < xadj[n]; ++e) g.adjwgt[e] = adjwgt ? adjwgt[e] : 1;
+    for (int v = 0; v < n; ++v) {
+        g.vwgt[v] = vwgt ? vwgt[v] : 1;
+        wmax = std::max<long long>(wmax, g.vwgt[v]);
+    }
+    std::vector<int> pv(part, part + n);
+    FmRefineParams fp;
+    fp.k = k;
+    fp.block_bound = bound;
+    fp.rounds = 10;
+    fp.seed = seed;
+    fp.pass_slack = wmax;
+    cut -= fm_refine_pairs(g, fp, pv);
+    std::copy(pv.begin(), pv.end(), part);
+    return cut;
+}
+
+// The drop-in entry points' common path: the preset of `mode` (an unknown mode falls back to
+// ECO, interface/kaHIP_interface.cpp:60-62), the pipeline, and the perfectly balanced post-pass
+// (kaHIP_interface.cpp:123-132). With imbalance 0 the pipeline runs with the bound
+// 1.01 ceil(W / k) (app/balance_configuration.h:41-43) and the post-pass enforces ceil(W / k);
+// with perfectly_balance it enforces (1 + imbalance) ceil(W / k).
+void dropin_call(const char* fn, int n, const int* vwgt, const int* xadj, const int* adjcwgt,
+                 const int* adjncy, int nparts, double imbalance, bool perfectly_balance,
+                 int seed, int mode, int* edgecut, int* part) {
+    if (mode < FAST || mode > STRONGSOCIAL) mode = ECO;
+    kahip_params p;
+    kahip_default_params(&p);
+    p.k = nparts;
+    p.imbalance = imbalance;
+    p.seed = seed;
+    // STRONGSOCIAL clusters with ensembles (app/configuration.h:584-591); the presets whose
+    // reference refinement is FM-based (ECO, STRONG, ECOSOCIAL, STRONGSOCIAL) add the CPU
+    // pairwise FM on the finest level after the LP refinement
+    if (mode == STRONGSOCIAL) p.ensemble = ensemble_size_for(p.k);
+    if (mode == ECO || mode == STRONG || mode == ECOSOCIAL || mode == STRONGSOCIAL)
+        p.fm_rounds = kPresetFmRounds;
+    if (run_kaffpa(n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr)) {
+        report(fn, edgecut);
+        return;
+    }
+    if (imbalance == 0 || perfectly_balance) {
+        long long W = 0;
+        for (int v = 0; v < n; ++v) W += vwgt ? vwgt[v] : 1;
+        const long long bound =
+            (long long)((1 + imbalance) * std::ceil((double)W / (double)nparts));
+        const long long cut = balance_postpass(n, xadj, adjncy, vwgt, adjcwgt, nparts, bound,
+                                               fm_seed(seed) ^ 0xBA1A'4CE5ULL, part);
+        if (edgecut) *edgecut = (int)cut;
+    }
 }
 
 }  // namespace
@@ -267,63 +386,54 @@ void kaffpa(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts
             double* imbalance, bool suppress_output, int seed, int mode, int* edgecut,
             int* part) {
     (void)suppress_output;  // ignored by the reference too (kaHIP_interface.cpp:108-113)
-    kahip_params p;
-    kahip_default_params(&p);
-    p.k = *nparts;
-    p.imbalance = *imbalance;
-    p.seed = seed;
-    // STRONGSOCIAL clusters with ensembles (app/configuration.h:584-591); the presets whose
-    // reference refinement is FM-based (ECO, STRONG, ECOSOCIAL, STRONGSOCIAL) add the CPU
-    // pairwise FM on the finest level after the LP refinement
-    if (mode == STRONGSOCIAL) p.ensemble = ensemble_size_for(p.k);
-    if (mode == ECO || mode == STRONG || mode == ECOSOCIAL || mode == STRONGSOCIAL)
-        p.fm_rounds = kPresetFmRounds;
-    if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
-        report("kaffpa", edgecut);
+    dropin_call("kaffpa", *n, vwgt, xadj, adjcwgt, adjncy, *nparts, *imbalance, false, seed,
+                mode, edgecut, part);
 }
 
 void kaffpa_balance(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts,
                     double* imbalance, bool perfectly_balance, bool suppress_output, int seed,
                     int mode, int* edgecut, int* part) {
     (void)suppress_output;
-    (void)mode;
-    kahip_params p;
-    kahip_default_params(&p);
-    p.k = *nparts;
-    p.imbalance = *imbalance;
-    p.seed = seed;
-    if (run_kaffpa(*n, vwgt, xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr)) {
-        report("kaffpa_balance", edgecut);
-        return;
-    }
-    if (perfectly_balance) {
-        long long W = 0;
-        for (int v = 0; v < *n; ++v) W += vwgt ? vwgt[v] : 1;
-        const long long bound =
-            (long long)((1 + *imbalance) * std::ceil((double)W / (double)*nparts));
-        *edgecut = (int)rebalance(*n, xadj, adjncy, vwgt, adjcwgt, *nparts, bound, part);
-    }
+    dropin_call("kaffpa_balance", *n, vwgt, xadj, adjcwgt, adjncy, *nparts, *imbalance,
+                perfectly_balance, seed, mode, edgecut, part);
 }
 
 void kaffpa_balance_NE(int* n, int* vwgt, int* xadj, int* adjcwgt, int* adjncy, int* nparts,
                        double* imbalance, bool suppress_output, int seed, int mode, int* edgecut,
                        int* part) {
     (void)suppress_output;
-    (void)mode;
-    // app/balance_configuration.h:27-37: node weight += weighted degree
-    std::vector<int> w(*n);
+    // app/balance_configuration.h:27-37: node weight += weighted degree, except in the perfectly
+    // balanced case (imbalance 0), which keeps the node weights
+    if (*imbalance == 0) {
+        dropin_call("kaffpa_balance_NE", *n, vwgt, xadj, adjcwgt, adjncy, *nparts, *imbalance,
+                    false, seed, mode, edgecut, part);
+        return;
+    }
+    std::vector<int> w(std::max(*n, 0));
     for (int v = 0; v < *n; ++v) {
         long long s = vwgt ? vwgt[v] : 1;
         for (int e = xadj[v]; e < xadj[v + 1]; ++e) s += adjcwgt ? adjcwgt[e] : 1;
+        if (s > INT_MAX) {
+            g_last_error = "kaffpa_balance_NE: node weight plus weighted degree exceeds INT_MAX";
+            report("kaffpa_balance_NE", edgecut);
+            return;
+        }
         w[v] = (int)s;
     }
-    kahip_params p;
-    kahip_default_params(&p);
-    p.k = *nparts;
-    p.imbalance = *imbalance;
-    p.seed = seed;
-    if (run_kaffpa(*n, w.data(), xadj, adjcwgt, adjncy, &p, edgecut, part, nullptr))
-        report("kaffpa_balance_NE", edgecut);
+    dropin_call("kaffpa_balance_NE", *n, w.data(), xadj, adjcwgt, adjncy, *nparts, *imbalance,
+                false, seed, mode, edgecut, part);
+}
+
+int kahip_rebalance(int n, const int* xadj, const int* adjncy, const int* vwgt,
+                    const int* adjwgt, int k, long long bound, unsigned long long seed,
+                    int* part, long long* cut) {
+    return guarded([&] {
+        check_csr(n, xadj, adjncy);
+        KH_REQUIRE(k >= 1, "k must be >= 1");
+        for (int v = 0; v < n; ++v) KH_REQUIRE(part[v] >= 0 && part[v] < k, "part out of range");
+        const long long c = balance_postpass(n, xadj, adjncy, vwgt, adjwgt, k, bound, seed, part);
+        if (cut) *cut = c;
+    });
 }
 
 int kahip_graph_from_host(int n, const int* xadj, const int* adjncy, const int* vwgt,

@@ -292,13 +292,18 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     S.ms_initial = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
     // ------------------------------------------------------------------ uncoarsening
-    EventPair ev_refine;
-    KH_CHECK(hipEventRecord(ev_refine.a, s));
+    // Every level: projection, LP refinement on the GPU (timed by an event pair per level) and,
+    // for the FM-based presets, the CPU pairwise FM of the level's partition, as the reference
+    // refines every level after projection (uncoarsening.cpp:58-95).
     DevBuf<int> W;
     W.reserve(p.k);
-    DevBuf<unsigned long long> trav;
+    HostGraph hl;
+    std::vector<int> hp;
+    S.ms_refine = 0;
     for (int l = level; l >= 0; --l) {
         const DGraph& gl = level_graph(l);
+        EventPair ev;
+        KH_CHECK(hipEventRecord(ev.a, s));
         int* part_l;
         if (l == level) {
             part_l = part_c.p;
@@ -314,32 +319,29 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
         // the profiler times the clustering and contraction kernels, not the refinement rounds
         lp.run(LpMode::Refine, gl, part_l, W.p, (int)U, p.refine_rounds, rseed, s, nullptr,
                "lp_refine");
+        KH_CHECK(hipEventRecord(ev.b, s));
         lp.check(s);
         S.edges_refine += lp.traversed(s);
-    }
-    KH_CHECK(hipEventRecord(ev_refine.b, s));
-    KH_CHECK(hipStreamSynchronize(s));
-    S.ms_refine = ev_refine.ms();
-
-    // ------------------------------------------------------------------ CPU pairwise FM
-    if (p.fm_rounds > 0 && g0.n > 0) {
-        const auto t1 = std::chrono::steady_clock::now();
-        HostGraph h0;
-        download_graph(g0, h0, s);
-        std::vector<int> hp(g0.n);
-        KH_CHECK(hipMemcpyAsync(hp.data(), part_c.p, sizeof(int) * g0.n, hipMemcpyDeviceToHost,
-                                s));
-        KH_CHECK(hipStreamSynchronize(s));
-        FmRefineParams fp;
-        fp.k = p.k;
-        fp.block_bound = U;
-        fp.rounds = p.fm_rounds;
-        fp.seed = fm_seed(p.seed);
-        S.fm_gain = fm_refine_pairs(h0, fp, hp);
-        KH_CHECK(hipMemcpyAsync(part_c.p, hp.data(), sizeof(int) * g0.n, hipMemcpyHostToDevice,
-                                s));
-        KH_CHECK(hipStreamSynchronize(s));
-        S.ms_fm = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        S.ms_refine += ev.ms();
+        if (p.fm_rounds > 0 && gl.n > 0) {
+            const auto t1 = std::chrono::steady_clock::now();
+            download_graph(gl, hl, s);
+            hp.resize(gl.n);
+            KH_CHECK(hipMemcpyAsync(hp.data(), part_l, sizeof(int) * gl.n, hipMemcpyDeviceToHost,
+                                    s));
+            KH_CHECK(hipStreamSynchronize(s));
+            FmRefineParams fp;
+            fp.k = p.k;
+            fp.block_bound = U;
+            fp.rounds = p.fm_rounds;
+            fp.seed = fm_seed(p.seed) + (uint64_t)l;
+            S.fm_gain += fm_refine_pairs(hl, fp, hp);
+            KH_CHECK(hipMemcpyAsync(part_l, hp.data(), sizeof(int) * gl.n, hipMemcpyHostToDevice,
+                                    s));
+            KH_CHECK(hipStreamSynchronize(s));
+            S.ms_fm += std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t1).count();
+        }
     }
 
     // ------------------------------------------------------------------ metrics

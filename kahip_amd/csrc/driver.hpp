@@ -30,11 +30,8 @@ struct PartitionParams {
     int ip_repetitions = -1;   // -1: the reference's count (initial_partitioning.cpp:69-81)
     int ip_threads = 0;
     int ip_tries = 32;         // BFS-grown starts per bisection of the initial partitioner
-    int fm_rounds = 0;         // > 0: CPU pairwise FM on the finest level after LP refinement
-    // 1: each level's refinement rounds as one captured hipGraph. Off by default: capture and
-    // instantiation cost more than the launch overhead they save (RGG 2^15..2^22, R-MAT 2^20 and
-    // 2^24 measured).
-    int graph_refine = 0;
+    int fm_rounds = 0;         // > 0: CPU pairwise FM on every level after its LP refinement
+    int graph_refine = 0;      // must be 0: hipGraph-captured refinement rounds are disabled
     BinConfig bins;
     bool profile = false;
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels
@@ -60,7 +57,7 @@ struct PartitionStats {
     double ms_coarsen = 0, ms_initial = 0, ms_refine = 0, ms_cut = 0;
     double ms_rating = 0;  // part of ms_coarsen
     bool reordered = false;
-    double ms_fm = 0;      // CPU pairwise FM on the finest level
+    double ms_fm = 0;      // CPU pairwise FM, all levels
     long long fm_gain = 0;
     long long edges_cluster = 0, edges_contract = 0, edges_refine = 0, edges_cut = 0;
     long long max_block_weight = 0;
@@ -96,7 +93,8 @@ unsigned cluster_bound_for(unsigned upper, int factor);
 // initial_partitioning.cpp:69-81 for the FASTSOCIAL preset (16 repetitions, minipreps 4).
 int ip_repetitions_for(int k);
 constexpr int kPresetFmRounds = 3;  // pairwise FM passes of the FM-based kaffpa presets
-// seed of the finest-level pairwise FM: (uint32 seed) x 0x9E3779B97F4A7C15 ^ 0xF3F3F3F3
+// seed of the pairwise FM of a partition with this seed (level l uses fm_seed(seed) + l):
+// (uint32 seed) x 0x9E3779B97F4A7C15 ^ 0xF3F3F3F3
 inline uint64_t fm_seed(int seed) {
     return ((uint64_t)(uint32_t)seed * 0x9E3779B97F4A7C15ULL) ^ 0xF3F3F3F3ULL;
 }

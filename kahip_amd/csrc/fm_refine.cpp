@@ -88,6 +88,7 @@ struct PairFm {
     std::vector<int>& part;
     std::vector<long long>& W;
     long long U;
+    long long slack = 0;
     std::vector<long long> gain;
     std::vector<uint64_t> key;
     std::vector<int> pos;
@@ -164,7 +165,7 @@ struct PairFm {
             const int v1 = heap[1].h.empty() ? -1 : heap[1].h[0];
             auto ok = [&](int v, int s) {
                 if (v < 0) return false;
-                return W[blk[1 - s]] + g.vwgt[v] <= U || W[blk[s]] > U;
+                return W[blk[1 - s]] + g.vwgt[v] <= U + slack || W[blk[s]] > U;
             };
             const bool a0 = ok(v0, 0), a1 = ok(v1, 1);
             int v = -1, s = 0;
@@ -241,6 +242,7 @@ long long fm_refine_pairs(const HostGraph& g, const FmRefineParams& p, std::vect
     std::vector<long long> W(k, 0);
     for (int v = 0; v < n; ++v) W[part[v]] += g.vwgt[v];
     PairFm fm(g, part, W, p.block_bound);
+    fm.slack = p.pass_slack;
     std::vector<int> stamp(k, -1);
     long long total = 0;
     for (int r = 0; r < p.rounds; ++r) {

@@ -20,6 +20,9 @@ struct FmRefineParams {
                                 // preset's percentage; 25 here, as FM runs on the finest level
                                 // only: 5 gave 0.5-1 % less gain on the delaunay/rgg fixtures)
     uint64_t seed = 0;
+    long long pass_slack = 0;   // within a pass a move may take its target up to U + pass_slack
+                                // (so that a balanced pair can exchange nodes); only prefixes
+                                // whose overload is no higher than at the pass start are kept
 };
 
 // Refines part in place; returns the cut reduction, which is >= 0 when every block respects U

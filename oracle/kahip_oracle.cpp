@@ -243,9 +243,12 @@ double orc_balance(int n, const int* vwgt, int k, const int* part) {
 // fraction through `imbalance = 100 * fraction` then `epsilon = imbalance / 100.0`
 // (interface/kaHIP_interface.cpp:115), which this reproduces in the same double operations.
 unsigned orc_upper_bound(int64_t total_weight, int k, double imbalance) {
+    // app/balance_configuration.h:39-46: imbalance 0 is the perfectly balanced mode, whose
+    // partitioning bound has 1 % of slack
     double imb_percent = 100 * imbalance;
     double epsilon = imb_percent / 100.0;
     unsigned load = (unsigned)total_weight;
+    if (imb_percent == 0) return (unsigned)((1 + epsilon + 0.01) * std::ceil(load / (double)k));
     return (unsigned)((1 + epsilon) * std::ceil(load / (double)k));
 }
 

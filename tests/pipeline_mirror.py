@@ -96,7 +96,8 @@ def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0
     coarsest graph (the library's kahip_amd.initial_partition by default). `scores(xadj, adjncy,
     vwgt, adjwgt, B, rating_seed)` (optional) gives the int32 edge scores that replace the edge
     weights in the clustering of levels < rating_levels. fm_rounds > 0 applies the library's CPU
-    pairwise FM (kahip_amd.fm_refine) to the finest level at the end, as the pipeline does."""
+    pairwise FM (kahip_amd.fm_refine) to every level after its LP refinement, as the pipeline
+    does (seed fm_seed(seed) + level)."""
     if ip is None:
         import kahip_amd
         ip = kahip_amd.initial_partition
@@ -139,11 +140,10 @@ def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0
             part = O.project(maps[l], part)
         rs = O.hash3(seed, 0x4E, l)
         part, _ = O.lp_refine_sync(x, a, vw, ew, k, U, refine_rounds, subrounds, rs, part)
-    if fm_rounds > 0 and n > 0:
-        import kahip_amd
-        x, a, vw, ew = levels[0]
-        part, _ = kahip_amd.fm_refine(x, a, vw, ew, k, U, part, rounds=fm_rounds,
-                                      seed=kahip_amd.fm_seed(seed))
+        if fm_rounds > 0 and len(x) > 1:
+            import kahip_amd
+            part, _ = kahip_amd.fm_refine(x, a, vw, ew, k, U, part, rounds=fm_rounds,
+                                          seed=(kahip_amd.fm_seed(seed) + l) & (2**64 - 1))
     info = {"U": U, "B": B, "levels": level,
             "level_n": [len(lv[0]) - 1 for lv in levels], "initial": initial,
             "coarsest": levels[level]}

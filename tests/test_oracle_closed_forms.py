@@ -160,3 +160,11 @@ def test_planted_cut_mistake_is_caught():
     part = np.random.default_rng(0).integers(0, 3, len(x) - 1).astype(np.int32)
     assert O.edge_cut(x, a, ew, part) * 2 != O.edge_cut(x, a, ew, part)
     assert cut_numpy(x, a, ew, np.zeros_like(part)) == 0
+
+
+def test_upper_bound_perfectly_balanced_mode():
+    """app/balance_configuration.h:41-43: imbalance 0 partitions under 1.01 ceil(W/k)."""
+    assert O.upper_bound(32768, 2, 0.0) == int(1.01 * 16384) == 16547
+    assert O.upper_bound(32768, 7, 0.0) == int(1.01 * 4682)
+    assert O.upper_bound(1000, 3, 0.0) == int(1.01 * 334)
+    assert O.upper_bound(32768, 2, 0.03) == 16875
