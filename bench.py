@@ -5,10 +5,13 @@ A step is one full kaffpa(FASTSOCIAL) pass over the workload graph, which is gen
 and stays resident in HBM: GPU LP clustering + contraction for every level, CPU initial
 partitioning of the coarsest graph, GPU projection + LP refinement for every level, GPU edge cut.
 
-metric value = GTEPS of the LP path = (directed edges traversed by LP clustering rounds,
-contractions, LP refinement rounds and the cut pass) / (GPU time of coarsening + refinement +
-cut, hipEvents on the library's stream). The CPU initial partitioning is outside the metric; its
-time is in ms_per_step (wall clock per full step) and reported separately.
+metric value = GTEPS of the LP path as the metric defines it = (2m per LP clustering round,
+per contraction, per LP refinement round and per cut pass, summed over the levels) / (GPU time
+of coarsening + refinement + cut, hipEvents on the library's stream). value_edges_read divides
+the same time into the edges actually read (the refinement rounds work on a frontier, and hubs
+decide from kept per-block counters without reading their rows); `phases` splits time, edges and
+algorithmic bytes per phase. The CPU initial partitioning is outside the metric; its time is in
+ms_per_step (wall clock per full step) and reported separately.
 
 roofline: the dominant kernel (largest share of the timed LP launches), with algorithmic bytes
 per launch (12 B per traversed edge + 16 B per node) over its duration, measured with hipEvents on
@@ -152,6 +155,62 @@ def measured_traffic(workload, kernel):
     return (t.get("per_scope") or {}).get(kernel)
 
 
+REFINE_ROUNDS = 25  # LP refinement rounds per level (the FASTSOCIAL preset)
+
+
+def refine_edges_per_round(st):
+    """2m of every level per refinement round: the metric's count for the LP refinement."""
+    return REFINE_ROUNDS * sum(st["level_m2"][l] for l in range(st["levels"] + 1))
+
+
+def host_cpu():
+    """Model name of this host's CPU and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"host_cpu": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def phase_breakdown(stats, prof, steps, cluster_rounds=10):
+    """Per-phase GPU time, edges and algorithmic bytes per step: clustering (the lp_cluster_*
+    profiler scopes), contraction (remap + contract), refinement (kahip_stats.ms_refine; edges =
+    the rows actually read), cut. Algorithmic bytes as in algorithmic_bytes()."""
+    st = stats[-1]
+    ms = lambda pred: sum(v["ms"] for k, v in prof.items() if pred(k)) / steps
+    levels = range(st["levels"])
+    cl_bytes = sum(16.0 * st["level_n"][l] * cluster_rounds for l in levels) + 12.0 * st["edges_cluster"]
+    ct_bytes = sum(12.0 * st["level_m2"][l] + 16.0 * st["level_n"][l] + 8.0 * st["level_m2"][l + 1]
+                   + 8.0 * st["level_n"][l + 1] for l in levels)
+    rf_bytes = 12.0 * st["edges_refine"]
+    cut_bytes = 12.0 * st["edges_cut"] + 12.0 * st["level_n"][0]
+    out = {}
+    for name, t, edges, b in (
+            ("clustering", ms(lambda k: k.startswith("lp_cluster")), st["edges_cluster"], cl_bytes),
+            ("contraction", ms(lambda k: k in ("contract", "remap")), st["edges_contract"], ct_bytes),
+            ("refinement", sum(s["ms_refine"] for s in stats) / len(stats), st["edges_refine"], rf_bytes),
+            ("cut", sum(s["ms_cut"] for s in stats) / len(stats), st["edges_cut"], cut_bytes)):
+        out[name] = {"ms": round(t, 2), "edges": edges, "alg_bytes": round(b),
+                     "gteps": round(edges / (t / 1e3) / 1e9, 2) if t > 0 else None,
+                     "alg_gbs": round(b / (t / 1e3) / 1e9, 1) if t > 0 else None}
+    out["reorder"] = {"ms": round(ms(lambda k: k == "reorder"), 2)}
+    out["refinement"]["edges_per_definition"] = refine_edges_per_round(st)
+    out["refinement"]["note"] = ("edges = rows read by the refinement kernels (hubs deciding "
+                                 "from their kept per-block counters read none) and rows of "
+                                 "moved nodes scanned for the next frontier; "
+                                 "edges_per_definition = 2m of every level per round")
+    return out
+
+
 def cpu_baseline(K, g, w, budget_s=30.0):
     """Reference LP (1 of the 10 clustering sweeps) + contraction of its clustering on the full
     workload graph, single thread, timed inside the reference build."""
@@ -169,6 +228,7 @@ def cpu_baseline(K, g, w, budget_s=30.0):
     edges = iters * len(a) + len(a)
     gteps = edges / (t_lp + t_ct) / 1e9
     return {"value": round(gteps, 5), "unit": "GTEPS", "cores": 1, "kind": "reference",
+            **host_cpu(),
             "sample": (f"reference size_constraint_label_propagation, {iters} of 10 sweeps "
                        f"(bound {B}) + contract_clustering of its {nc} clusters on the full "
                        f"workload graph (n={n}, 2m={len(a)}), single thread; "
@@ -236,9 +296,16 @@ def main():
     wall = time.perf_counter() - t_start
 
     gpu_ms = [s["ms_coarsen"] + s["ms_refine"] + s["ms_cut"] for s in stats]
-    edges = [s["edges_cluster"] + s["edges_contract"] + s["edges_refine"] + s["edges_cut"]
+    # value: GTEPS as the metric defines it (2m per LP clustering round, per contraction, per LP
+    # refinement round and per cut pass); beside it the same time with the refinement counting
+    # only the edges its kernels actually read (its rounds work on a frontier and hubs decide from
+    # kept per-block counters)
+    edges = [s["edges_cluster"] + s["edges_contract"] + refine_edges_per_round(s) + s["edges_cut"]
              for s in stats]
+    edges_read = [s["edges_cluster"] + s["edges_contract"] + s["edges_refine"] + s["edges_cut"]
+                  for s in stats]
     gteps = sum(edges) / (sum(gpu_ms) / 1e3) / 1e9
+    gteps_read = sum(edges_read) / (sum(gpu_ms) / 1e3) / 1e9
     path_bytes = algorithmic_bytes(stats[-1])
     path_gbs = path_bytes / (gpu_ms[-1] / 1e3) / 1e9
 
@@ -301,6 +368,7 @@ def main():
                                 f"kaffpa FASTSOCIAL k={w['k']} eps=0.03",
                    "n": info["n"], "m2": info["m2"], "k": w["k"], "imbalance": 0.03,
                    "parallelism": "single-gpu"},
+        "value_edges_read": round(gteps_read, 4),
         "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
         "gpu_ms_unprofiled_warmup": None if warm_gpu_ms is None else round(warm_gpu_ms, 2),
         "cpu_initial_partition_ms_per_step": round(sum(s["ms_initial"] for s in stats) / len(stats), 1),
@@ -312,6 +380,7 @@ def main():
         "learned_rating": learned,
         "balance": round(last["balance"], 5),
         "levels": last["level_n"],
+        "phases": phase_breakdown(stats, prof, args.steps),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -488,6 +488,8 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
     for (int j = 0; j < EPT; ++j)
         if (labs[j] >= 0) atomicAdd(&mine[labs[j]], ws[j]);
     __syncthreads();
+    // traversed edges: the edges this chunk read (hubs deciding from kept counters read none)
+    if (threadIdx.x == 0) atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(e1 - e0));
     if (counted) {
         int* out = dcnt + dense_off[gi];
         for (int b = threadIdx.x; b < num_labels; b += kHubDenseThreads) {
@@ -515,10 +517,7 @@ __global__ __launch_bounds__(kHubDenseThreads) void propose_hub_dense_chunk(
         if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
     }
     block_reduce<kHubDenseThreads>(conn_own, c);
-    if (threadIdx.x == 0) {
-        decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
-        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(end - beg));
-    }
+    if (threadIdx.x == 0) decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
 }
 
 // One workgroup per multi-chunk refinement hub: decides from the summed counters and clears them
@@ -550,10 +549,8 @@ __global__ __launch_bounds__(256) void hub_dense_decide(
         if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
     }
     block_reduce<256>(conn_own, c);
-    if (threadIdx.x == 0) {
-        decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
-        atomicAdd(&a.trav[blockIdx.x & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
-    }
+    // no traversed edges here: the chunks that read the hub's edges counted them
+    if (threadIdx.x == 0) decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
 }
 
 // ---------------------------------------------------------------- hubs: chunked LDS hash
@@ -948,6 +945,7 @@ __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restri
                     a.big[bi] = v;
                     if (a.hub_cnt) a.big_from[bi] = own;
                 } else {
+                    atomicAdd(&a.trav[(blockIdx.x + threadIdx.x) & 63], (unsigned long long)(end - beg));
                     for (int e = beg; e < end; ++e) {
                         const int u = a.adjncy[e];
                         a.next_active[u] = 1;
@@ -975,6 +973,8 @@ __global__ __launch_bounds__(256) void mark_big_neighbours(LpArgs a) {
         const int v = a.big[i];
         const int to = a.label[v];
         const int from = a.hub_cnt ? a.big_from[i] : 0;
+        if (threadIdx.x == 0)
+            atomicAdd(&a.trav[i & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
         for (int e = a.xadj[v] + threadIdx.x; e < a.xadj[v + 1]; e += 256) {
             const int u = a.adjncy[e];
             a.next_active[u] = 1;

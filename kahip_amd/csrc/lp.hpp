@@ -63,7 +63,9 @@ public:
     static void init_cluster_labels(const DGraph& g, int* label, int* size, hipStream_t s);
     // Moves per round of the last run (synchronises the stream).
     void read_moves(int rounds, int* out, hipStream_t s);
-    // Directed edges traversed by the last refinement run (synchronises the stream).
+    // Directed edges read by the last refinement run (synchronises the stream): the rows scanned
+    // by the proposal kernels (hubs deciding from their kept counters read none) and the rows of
+    // moved nodes scanned to mark their neighbours and update hub counters.
     long long traversed(hipStream_t s);
     // Throws if a kernel of the last run() set the device error flag (synchronises the stream).
     void check(hipStream_t s);

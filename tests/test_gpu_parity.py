@@ -296,8 +296,12 @@ def test_kaffpa_eco_adds_pairwise_fm(G):
     part, st = K.partition(x, a, vw, ew, k=8, imbalance=0.03, seed=2,
                            fm_rounds=K.PRESET_FM_ROUNDS)
     assert np.array_equal(np.array(blocks), part) and cut == st["cut"]
+    assert cut == O.edge_cut(x, a, ew, part)
+    # the FM runs on every level (its gains there carry to the finer levels, where the LP
+    # refinement continues from a different partition): gains are non-negative and it costs time
+    assert st["fm_gain"] > 0 and st["ms_fm"] > 0
     _, plain = K.partition(x, a, vw, ew, k=8, imbalance=0.03, seed=2)
-    assert st["fm_gain"] == plain["cut"] - st["cut"] and st["fm_gain"] >= 0
+    assert plain["fm_gain"] == 0 and plain["ms_fm"] == 0
 
 
 @pytest.mark.parametrize("name,k,seed", [("delaunay15", 8, 1), ("rgg15", 2, 2), ("rmat12", 16, 3),
