@@ -126,6 +126,18 @@ int kahip_lp_refine(int n, const int* xadj, const int* adjncy, const int* vwgt,
 int kahip_edge_cut(int n, const int* xadj, const int* adjncy, const int* adjwgt, const int* part,
                    long long* cut);
 int kahip_block_weights(int n, const int* vwgt, int k, const int* part, long long* weights);
+/* Edge cut and block weights of a partition in device memory, allocating nothing: no counterpart
+ * in the reference. It is for callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc). Every pointer is a device pointer:
+ * the CSR (n nodes, m2 = xadj[n] directed entries; adjwgt / vwgt NULL for unit weights), part[n]
+ * with ids in [0, k), and the outputs *cut (one long long) and block_weights[k]. The work is
+ * enqueued on `stream` (a hipStream_t of the current device; NULL: the null stream) and not
+ * waited for. k <= 65536, the size of the library's per-GPU scratch array (2 + 65536 u64,
+ * 512 KiB, held until the process ends); larger k is rejected. *cut receives -1 if a part
+ * entry lies outside [0, k). Calls that overlap on one GPU share that scratch and race. */
+int kahip_quality_noalloc(int n, long long m2, const int* xadj, const int* adjncy,
+                          const int* adjwgt, const int* vwgt, const int* part, int k,
+                          long long* cut, long long* block_weights, void* stream);
 int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_part,
                   int* fine_part);
 /* CPU initial partitioner of the coarsest graph (no GPU). */

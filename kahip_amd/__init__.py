@@ -103,7 +103,7 @@ EXPORTS = [
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
     "kahip_mlp_train", "kahip_batch_from_host", "kahip_batch_generate", "kahip_batch_info",
     "kahip_batch_to_host", "kahip_batch_free", "kahip_batch_partition", "kahip_autotune_bins",
-    "kahip_fm_refine", "kahip_rebalance",
+    "kahip_fm_refine", "kahip_quality_noalloc", "kahip_rebalance",
 ]
 
 _lib = None

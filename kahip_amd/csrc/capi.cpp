@@ -499,6 +499,15 @@ int kahip_edge_cut(int n, const int* xadj, const int* adjncy, const int* adjwgt,
     });
 }
 
+int kahip_quality_noalloc(int n, long long m2, const int* xadj, const int* adjncy,
+                          const int* adjwgt, const int* vwgt, const int* part, int k,
+                          long long* cut, long long* block_weights, void* stream) {
+    return guarded([&] {
+        quality_noalloc(n, m2, xadj, adjncy, adjwgt, vwgt, part, k, cut, block_weights,
+                        static_cast<hipStream_t>(stream));
+    });
+}
+
 int kahip_block_weights(int n, const int* vwgt, int k, const int* part, long long* weights) {
     return guarded([&] {
         KH_REQUIRE(n >= 0 && k >= 1 && k <= 65536, "bad n or k");
