@@ -47,7 +47,11 @@ WORKLOADS = {
 }
 
 
-BATCH = dict(graphs=4096, log2_min=10.0, log2_max=16.0, k=8, seed=1, micro=64)
+# ip_tries: BFS-grown starts per bisection of the CPU initial partitioner; 8 keeps the batched
+# cut at the reference's (geometric mean 1.003 on the first 40 graphs, tests/test_quality_batch.py)
+# for 2.7x less initial-partitioning time than the default 32 (0.967)
+BATCH = dict(graphs=4096, log2_min=10.0, log2_max=16.0, k=8, seed=1, micro=64, ip_tries=8,
+             single_sample=128)
 
 
 def run_batch(K, args):
@@ -61,14 +65,15 @@ def run_batch(K, args):
     inf = b.info()
     log(f"batch: {inf['graphs']} graphs, n={inf['n']} 2m={inf['m2']} generated in "
         f"{time.time() - t0:.1f}s")
-    bins, tune_ms = K.autotune_bins(b, k=w["k"])
-    log(f"autotuned bins {bins} (ms per candidate {[round(t, 2) for t in tune_ms]})")
+    tuned, tune_ms = K.autotune(b, k=w["k"])
+    tuned["ip_tries"] = w["ip_tries"]
+    log(f"autotuned {tuned} (ms per evaluation {[round(t, 2) for t in tune_ms]})")
     for i in range(args.warmup):
-        b.partition(k=w["k"], seed=0, bins=bins)
+        b.partition(k=w["k"], seed=0, **tuned)
     walls, stats = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        _, cuts, mbw, st = b.partition(k=w["k"], seed=0, bins=bins)
+        _, cuts, mbw, st = b.partition(k=w["k"], seed=0, **tuned)
         walls.append(time.perf_counter() - t)
         stats.append(st)
         log(f"step {i}: wall {walls[-1]*1e3:.1f} ms gpu {st['ms_coarsen'] + st['ms_refine'] + st['ms_cut']:.1f} ms "
@@ -85,12 +90,27 @@ def run_batch(K, args):
             gs.append((x[off[g]:off[g + 1] + 1] - e0, a[e0:e1] - off[g], None, ew[e0:e1]))
         mb = K.Batch.from_graphs(gs)
         t = time.perf_counter()
-        mb.partition(k=w["k"], seed=0, bins=bins)
+        mb.partition(k=w["k"], seed=0, **tuned)
         lat += [time.perf_counter() - t] * (hi - lo)
         mb.free()
+    # per-graph latency of graphs submitted one at a time (a sample of the batch)
+    single = []
+    for g in range(0, w["graphs"], w["graphs"] // w["single_sample"]):
+        e0, e1 = x[off[g]], x[off[g + 1]]
+        gx, ga, gw = x[off[g]:off[g + 1] + 1] - e0, a[e0:e1] - off[g], ew[e0:e1]
+        t = time.perf_counter()
+        K.partition(gx, ga, None, gw, k=w["k"], seed=0, **tuned)
+        single.append(time.perf_counter() - t)
     b.free()
     lat = np.array(lat) * 1e3
+    single = np.array(single) * 1e3
     gps = w["graphs"] / (sum(walls) / len(walls))
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = batch_cpu_baseline(w)
+        except Exception as e:  # reported, never silently replaced
+            log(f"batch cpu baseline failed: {e!r}")
     gpu = [s["ms_coarsen"] + s["ms_refine"] + s["ms_cut"] for s in stats]
     edges = [s["edges_cluster"] + s["edges_contract"] + s["edges_refine"] + s["edges_cut"]
              for s in stats]
@@ -101,15 +121,56 @@ def run_batch(K, args):
            "config": {"workload": (f"{w['graphs']} graphs, RGG/R-MAT alternating, n log-uniform in "
                                    f"[2^{w['log2_min']:.0f}, 2^{w['log2_max']:.0f}], k={w['k']} "
                                    f"eps=0.03, one disjoint union, LP rounds launched directly"),
-                      "n": inf["n"], "m2": inf["m2"], "bins": bins},
+                      "n": inf["n"], "m2": inf["m2"], "tuned": tuned},
            "gpu_ms_per_step": round(sum(gpu) / len(gpu), 2),
            "gpu_gteps": round(sum(edges) / (sum(gpu) / 1e3) / 1e9, 3),
            "cpu_initial_partition_ms_per_step": round(
                sum(s["ms_initial"] for s in stats) / len(stats), 1),
            "latency_ms_micro_batches": {"micro_batch": w["micro"],
                                         "p50": round(float(np.percentile(lat, 50)), 2),
-                                        "p99": round(float(np.percentile(lat, 99)), 2)}}
+                                        "p99": round(float(np.percentile(lat, 99)), 2)},
+           "latency_ms_single_graph": {"graphs": len(single),
+                                       "p50": round(float(np.percentile(single, 50)), 2),
+                                       "p99": round(float(np.percentile(single, 99)), 2)},
+           "cpu_baseline": cpu}
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = round(gps / cpu["value"], 2)
     print(json.dumps(out), flush=True)
+
+
+def _ref_batch_graph(args):
+    """Reference kaffpa(FASTSOCIAL) on graph i of the batch in this worker process."""
+    import oracle as O
+    from kahip_amd import graphs as GR
+    seed, i, k = args
+    x, a, vw, ew = GR.batch_graph(seed, i)
+    t = time.perf_counter()
+    O.ref_kaffpa(x, a, vw, ew, k, 0.03, 0, 3)
+    return time.perf_counter() - t
+
+
+def batch_cpu_baseline(w, sample=128):
+    """The reference's own kaffpa(FASTSOCIAL) (oracle/_ref) on a sample of the batch's graphs (every
+    graphs/sample-th), in P = usable-cores independent processes (its RNG is process-global), as
+    graphs/s; the single-process rate is reported beside it."""
+    import multiprocessing as mp
+    import oracle as O
+    if not O.ref_available():
+        return None
+    hc = host_cpu()
+    P = hc["usable_cores"] or hc["nproc"] or 1
+    idx = list(range(0, w["graphs"], w["graphs"] // sample))[:sample]
+    log(f"batch cpu baseline: reference kaffpa FASTSOCIAL on {len(idx)} graphs, {P} processes")
+    t = time.perf_counter()
+    with mp.get_context("spawn").Pool(P) as pool:
+        per = pool.map(_ref_batch_graph, [(w["seed"], i, w["k"]) for i in idx], chunksize=1)
+    wall = time.perf_counter() - t
+    return {"value": round(len(idx) / wall, 2), "unit": "graphs/s", "cores": P,
+            "kind": "reference", **hc,
+            "single_process_graphs_per_s": round(len(idx) / sum(per), 3),
+            "sample": (f"reference kaffpa(FASTSOCIAL, k={w['k']}, eps=0.03, seed 0) on {len(idx)} "
+                       f"of the {w['graphs']} graphs (every {w['graphs'] // sample}-th), "
+                       f"{P} independent processes, wall {wall:.1f}s")}
 
 
 def log(msg):

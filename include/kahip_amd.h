@@ -43,6 +43,10 @@ typedef struct {
     int graph_refine;    /* must be 0 (the default): hipGraph-captured refinement rounds are
                             disabled after an unexplained host fault in that path under a
                             profiler; non-zero is rejected with an error */
+    int group_items;     /* tile shape: nodes per lane group in flight in the 8..64-lane LP
+                            kernels, 1, 2 (default) or 4 */
+    int wave_small;      /* tile shape: clustering wave-bin nodes up to this degree use the
+                            small-table wave kernel, 64, 128 (default) or 256 */
 } kahip_params;
 
 /* Edge ratings of the clustering. NONE uses the edge weights (the FASTSOCIAL preset);
@@ -140,10 +144,12 @@ int kahip_quality_noalloc(int n, long long m2, const int* xadj, const int* adjnc
                           long long* cut, long long* block_weights, void* stream);
 int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_part,
                   int* fine_part);
-/* CPU initial partitioner of the coarsest graph (no GPU). */
+/* CPU initial partitioner of the coarsest graph (no GPU). repetitions <= 0 and
+ * bisection_tries <= 0 select the defaults (the reference's repetition count; 32 BFS-grown
+ * starts per bisection, kahip_params.ip_tries). */
 int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
                             const int* adjwgt, int k, long long block_bound, int repetitions,
-                            unsigned long long seed, int* part);
+                            int bisection_tries, unsigned long long seed, int* part);
 
 /* CPU post-pass of the perfectly balanced modes (the reference runs cycle_refinement there,
  * interface/kaHIP_interface.cpp:123-132): moves nodes out of blocks heavier than `bound`, best
@@ -234,6 +240,15 @@ int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_g
 #define KAHIP_AUTOTUNE_CANDIDATES 6
 int kahip_autotune_bins(kahip_graph_t g, kahip_batch_t b, const kahip_params* params, int* bins,
                         double* ms_per_candidate);
+/* Autotuner of the LP kernels' degree bins and tile shapes for a device graph or batch (exactly
+ * one non-NULL), by coordinate descent, each candidate timed on two LP clustering rounds: the
+ * wavefront bin's upper degree (bins[4] in 96..512), then group_items (1, 2, 4), then wave_small
+ * (64, 128, 256). *tuned receives *params (or the defaults) with bins, group_items and wave_small
+ * set to the fastest configuration; the LP results do not depend on them. ms (may be NULL)
+ * receives KAHIP_AUTOTUNE_EVALS timings in that order. */
+#define KAHIP_AUTOTUNE_EVALS 12
+int kahip_autotune(kahip_graph_t g, kahip_batch_t b, const kahip_params* params,
+                   kahip_params* tuned, double* ms);
 
 #ifdef __cplusplus
 }

@@ -52,7 +52,7 @@ class Params(C.Structure):
                 ("profile", C.c_int), ("rating", C.c_int), ("rating_levels", C.c_int),
                 ("mlp_weights", C.POINTER(C.c_float)), ("reorder", C.c_int),
                 ("ensemble", C.c_int), ("ip_tries", C.c_int), ("fm_rounds", C.c_int),
-                ("graph_refine", C.c_int)]
+                ("graph_refine", C.c_int), ("group_items", C.c_int), ("wave_small", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -103,7 +103,7 @@ EXPORTS = [
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
     "kahip_mlp_train", "kahip_batch_from_host", "kahip_batch_generate", "kahip_batch_info",
     "kahip_batch_to_host", "kahip_batch_free", "kahip_batch_partition", "kahip_autotune_bins",
-    "kahip_fm_refine", "kahip_quality_noalloc", "kahip_rebalance",
+    "kahip_fm_refine", "kahip_quality_noalloc", "kahip_rebalance", "kahip_autotune",
 ]
 
 _lib = None
@@ -159,7 +159,7 @@ def lib():
     L.kahip_block_weights.argtypes = [C.c_int, ip, C.c_int, _i32p, _i64p]
     L.kahip_project.argtypes = [C.c_int, _i32p, C.c_int, _i32p, _i32p]
     L.kahip_initial_partition.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
-                                          C.c_int, C.c_ulonglong, _i32p]
+                                          C.c_int, C.c_int, C.c_ulonglong, _i32p]
     L.kahip_rebalance.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong,
                                   C.c_ulonglong, _i32p, C.POINTER(C.c_longlong)]
     L.kahip_fm_refine.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_longlong, C.c_int,
@@ -185,6 +185,8 @@ def lib():
                                         C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                         C.POINTER(BatchStats)]
     L.kahip_autotune_bins.argtypes = [G, G, C.POINTER(Params), ip, C.POINTER(C.c_double)]
+    L.kahip_autotune.argtypes = [G, G, C.POINTER(Params), C.POINTER(Params),
+                                 C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -422,14 +424,16 @@ def project(mapping, coarse_part):
     return out[:len(mapping)]
 
 
-def initial_partition(xadj, adjncy, vwgt, adjwgt, k, block_bound, repetitions=-1, seed=0):
-    """The CPU initial partitioner of the coarsest graph (runs without a GPU)."""
+def initial_partition(xadj, adjncy, vwgt, adjwgt, k, block_bound, repetitions=-1, seed=0,
+                      tries=-1):
+    """The CPU initial partitioner of the coarsest graph (runs without a GPU); tries = BFS-grown
+    starts per bisection (-1: the default, kahip_params.ip_tries)."""
     xadj, adjncy = _i32(xadj), _i32(adjncy)
     n = len(xadj) - 1
     out = np.zeros(max(n, 1), np.int32)
     _check(lib().kahip_initial_partition(n, xadj, adjncy if len(adjncy) else np.zeros(1, np.int32),
                                          _p(vwgt), _p(adjwgt), k, int(block_bound), repetitions,
-                                         seed, out))
+                                         tries, seed, out))
     return out[:n]
 
 
@@ -632,3 +636,20 @@ def autotune_bins(target, k=8, **params):
     _check(lib().kahip_autotune_bins(g, b, C.byref(p), bins.ctypes.data_as(C.POINTER(C.c_int)),
                                      ms.ctypes.data_as(C.POINTER(C.c_double))))
     return bins.tolist(), ms.tolist()
+
+
+AUTOTUNE_EVALS = 12
+
+
+def autotune(target, k=8, **params):
+    """Autotuned LP degree bins and tile shapes for a Graph or Batch: (dict with bins,
+    group_items, wave_small; ms per evaluation). Pass the dict's entries to partition()."""
+    ms = np.zeros(AUTOTUNE_EVALS, np.float64)
+    p = default_params(k=k, **params)
+    out = Params()
+    g = target._h if isinstance(target, Graph) else None
+    b = target._h if isinstance(target, Batch) else None
+    _check(lib().kahip_autotune(g, b, C.byref(p), C.byref(out),
+                                ms.ctypes.data_as(C.POINTER(C.c_double))))
+    return ({"bins": list(out.bins), "group_items": out.group_items,
+             "wave_small": out.wave_small}, ms.tolist())

@@ -149,8 +149,10 @@ void run_lp(LpWorkspace& lp, LpMode mode, const DGraph& g, int* label, int* size
 
 }  // namespace
 
-BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, double* ms) {
-    static const int cand[kAutotuneCandidates] = {96, 128, 192, 256, 384, 512};
+BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, bool shapes, double* ms) {
+    static const int wave_hi[kAutotuneCandidates] = {96, 128, 192, 256, 384, 512};
+    static const int items[3] = {1, 2, 4};
+    static const int small[3] = {64, 128, 256};
     hipStream_t s;
     KH_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     LpWorkspace lp;
@@ -159,13 +161,11 @@ BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, double* ms) {
     size.reserve(std::max(g.n, 1));
     const unsigned U = upper_bound_for(g.total_vwgt, p.k, p.imbalance);
     const int B = (int)cluster_bound_for(U, p.cluster_factor);
-    BinConfig best = p.bins;
-    float best_ms = 1e30f;
+    const uint32_t seed = cluster_seed(p.seed, 0);
     Ev ev;
-    for (int c = 0; c < kAutotuneCandidates; ++c) {
-        BinConfig bc = p.bins;
-        bc.hi[kNumBins - 2] = cand[c];
-        const uint32_t seed = cluster_seed(p.seed, 0);
+    int evals = 0;
+    // two LP clustering rounds under bc, timed
+    auto time_config = [&](const BinConfig& bc) {
         LpWorkspace::init_cluster_labels(g, label.p, size.p, s);
         lp.prepare(LpMode::Cluster, g, g.n, p.subrounds, seed, bc, s);
         KH_CHECK(hipEventRecord(ev.a, s));
@@ -174,10 +174,43 @@ BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, double* ms) {
         KH_CHECK(hipStreamSynchronize(s));
         lp.check(s);
         const float t = ev.ms();
-        if (ms) ms[c] = t;
+        if (ms) ms[evals] = t;
+        ++evals;
+        return t;
+    };
+    // coordinate descent: the wave bin's upper degree, then (with shapes) the group kernels'
+    // items per lane group, then the small wave kernel's degree limit
+    BinConfig best = p.bins;
+    float best_ms = 1e30f;
+    for (int c = 0; c < kAutotuneCandidates; ++c) {
+        BinConfig bc = best;
+        bc.hi[kNumBins - 2] = wave_hi[c];
+        const float t = time_config(bc);
         if (t < best_ms) {
             best_ms = t;
             best = bc;
+        }
+    }
+    if (shapes) {
+        const BinConfig base = best;
+        for (int c = 0; c < 3; ++c) {
+            BinConfig bc = base;
+            bc.group_items = items[c];
+            const float t = time_config(bc);
+            if (t < best_ms) {
+                best_ms = t;
+                best = bc;
+            }
+        }
+        const BinConfig base2 = best;
+        for (int c = 0; c < 3; ++c) {
+            BinConfig bc = base2;
+            bc.wave_small = small[c];
+            const float t = time_config(bc);
+            if (t < best_ms) {
+                best_ms = t;
+                best = bc;
+            }
         }
     }
     KH_CHECK(hipStreamDestroy(s));

@@ -44,10 +44,13 @@ void upload_batch(int G, const long long* node_off, const long long* edge_off, c
 void generate_batch(int G, double log2_min, double log2_max, unsigned long long seed,
                     OwnedGraph& out, std::vector<int>& node_off);
 
-// Times two LP clustering rounds for each candidate upper degree of the wavefront bin
-// (kAutotuneCandidates values, the rest of the bins fixed) on g and returns the fastest
-// configuration; ms (optional) receives each candidate's time.
+// Times two LP clustering rounds per candidate configuration on g and returns the fastest, by
+// coordinate descent: the wavefront bin's upper degree (kAutotuneCandidates values), then, with
+// shapes, the group kernels' nodes per lane group (1, 2, 4) and the small wave kernel's degree
+// limit (64, 128, 256). ms (optional) receives each evaluation's time in that order
+// (kAutotuneCandidates, or kAutotuneEvals with shapes).
 constexpr int kAutotuneCandidates = 6;
-BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, double* ms);
+constexpr int kAutotuneEvals = kAutotuneCandidates + 6;
+BinConfig autotune_bins(const DGraph& g, const PartitionParams& p, bool shapes, double* ms);
 
 }  // namespace kahip

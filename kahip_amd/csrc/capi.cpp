@@ -67,6 +67,12 @@ PartitionParams to_params(const kahip_params* q) {
     p.ip_repetitions = q->ip_repetitions;
     p.ip_threads = q->ip_threads;
     for (int b = 0; b < kNumBins; ++b) p.bins.hi[b] = q->bins[b];
+    KH_REQUIRE(q->group_items == 1 || q->group_items == 2 || q->group_items == 4,
+               "group_items must be 1, 2 or 4");
+    KH_REQUIRE(q->wave_small == 64 || q->wave_small == 128 || q->wave_small == 256,
+               "wave_small must be 64, 128 or 256");
+    p.bins.group_items = q->group_items;
+    p.bins.wave_small = q->wave_small;
     p.profile = q->profile != 0;
     KH_REQUIRE(q->rating >= 0 && q->rating <= 2, "bad rating");
     p.rating = (RatingMode)q->rating;
@@ -223,6 +229,8 @@ void kahip_default_params(kahip_params* p) {
     p->ip_repetitions = d.ip_repetitions;
     p->ip_threads = d.ip_threads;
     for (int b = 0; b < kNumBins; ++b) p->bins[b] = d.bins.hi[b];
+    p->group_items = d.bins.group_items;
+    p->wave_small = d.bins.wave_small;
     p->profile = 0;
     p->rating = (int)d.rating;
     p->rating_levels = d.rating_levels;
@@ -560,7 +568,7 @@ int kahip_project(int n_fine, const int* map, int n_coarse, const int* coarse_pa
 
 int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int* vwgt,
                             const int* adjwgt, int k, long long block_bound, int repetitions,
-                            unsigned long long seed, int* part) {
+                            int bisection_tries, unsigned long long seed, int* part) {
     return guarded([&] {
         check_csr(n, xadj, adjncy);
         KH_REQUIRE(k >= 1, "k must be >= 1");
@@ -577,6 +585,7 @@ int kahip_initial_partition(int n, const int* xadj, const int* adjncy, const int
         p.block_bound = block_bound;
         p.repetitions = repetitions > 0 ? repetitions : ip_repetitions_for(k);
         p.seed = seed;
+        if (bisection_tries > 0) p.bisection_tries = bisection_tries;
         std::vector<int> out;
         initial_partition(g, p, out);
         std::copy(out.begin(), out.end(), part);
@@ -858,8 +867,26 @@ int kahip_autotune_bins(kahip_graph_t g, kahip_batch_t b, const kahip_params* pa
         KH_REQUIRE((g != nullptr) != (b != nullptr), "pass exactly one of graph and batch");
         PartitionParams p = to_params(params);
         const DGraph& d = g ? g->og.g : b->og.g;
-        BinConfig best = autotune_bins(d, p, ms_per_candidate);
+        BinConfig best = autotune_bins(d, p, false, ms_per_candidate);
         for (int i = 0; i < kNumBins; ++i) bins[i] = best.hi[i];
+    });
+}
+
+int kahip_autotune(kahip_graph_t g, kahip_batch_t b, const kahip_params* params,
+                   kahip_params* tuned, double* ms) {
+    return guarded([&] {
+        KH_REQUIRE((g != nullptr) != (b != nullptr), "pass exactly one of graph and batch");
+        KH_REQUIRE(tuned != nullptr, "tuned must not be NULL");
+        PartitionParams p = to_params(params);
+        const DGraph& d = g ? g->og.g : b->og.g;
+        const BinConfig best = autotune_bins(d, p, true, ms);
+        kahip_params out;
+        if (params) out = *params;
+        else kahip_default_params(&out);
+        for (int i = 0; i < kNumBins; ++i) out.bins[i] = best.hi[i];
+        out.group_items = best.group_items;
+        out.wave_small = best.wave_small;
+        *tuned = out;
     });
 }
 

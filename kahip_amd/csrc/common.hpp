@@ -113,6 +113,11 @@ enum class LpMode : int { Cluster = 0, Refine = 1 };
 constexpr int kNumBins = 6;
 struct BinConfig {
     int hi[kNumBins] = {8, 16, 32, 64, 512, 0x7fffffff};
+    // Launch (tile) shapes of the LP kernels; the synchronous schedule makes the result
+    // independent of them.
+    int group_items = 2;   // nodes per lane group in flight in the 8..64-lane kernels: 1, 2 or 4
+    int wave_small = 128;  // clustering: wave-bin nodes up to this degree use the small-table wave
+                           // kernel (2 x wave_small LDS slots per wave): 64, 128 or 256
 };
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

@@ -32,12 +32,12 @@ namespace {
 constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => load <= 1/2)
 static_assert(kWaveCap == 1024, "propose_wave loads up to kWaveCap / 2 edges per wave");
 constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
-// Clustering splits the wave list: nodes of degree <= kWaveSmall come first and use a kernel of
-// 256 slots per wave (8 KB of LDS per workgroup instead of 32 KB, so more waves per CU).
-constexpr int kWaveSmall = 128;
+// Clustering splits the wave list: nodes of degree <= BinConfig::wave_small (64, 128 or 256)
+// come first and use a kernel of 2 x wave_small slots per wave (8 KB of LDS per workgroup at 128
+// instead of 32 KB, so more waves per CU).
 constexpr int kBlock = 256;
 constexpr int kFrontierGrid = 2048;  // workgroups of a refinement list kernel (grid-stride)
-constexpr int kGroupItems = 2;       // nodes per lane group in flight in propose_group
+// nodes per lane group in flight in propose_group: BinConfig::group_items (1, 2 or 4)
 constexpr int kHubChunk = 4096;   // hub edges per workgroup
 constexpr int kHubCap = 8192;     // LDS hash slots per hub chunk (load <= 1/2)
 constexpr int kHubThreads = 1024;      // clustering hub chunks (4 edges per thread)
@@ -183,13 +183,12 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
 // ---------------------------------------------------------------- G lanes per node (deg <= G)
 // dcount: the list's length on the device (the compacted frontier of a refinement round);
 // workgroups then stride over the list's virtual blocks, so a small frontier costs few of them.
-// A virtual block holds kGroupItems x (kBlock / G) nodes: every lane group serves kGroupItems
-// nodes at once, whose dependent loads (list, offsets, targets, labels, label weights) are
-// issued together, so each wave has that many chains in flight.
-template <LpMode M, int G>
+// A virtual block holds IT x (kBlock / G) nodes: every lane group serves IT nodes at once, whose
+// dependent loads (list, offsets, targets, labels, label weights) are issued together, so each
+// wave has that many chains in flight.
+template <LpMode M, int G, int IT>
 __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
                                                         int count, const int* __restrict__ dcount) {
-    constexpr int IT = kGroupItems;
     constexpr int NB = kBlock / G;  // nodes per item and workgroup
     if (dcount) count = *dcount;
     const int nvb = (int)(((long long)count + IT * NB - 1) / (IT * NB));
@@ -284,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
     constexpr int HC = 2 * IT * NB;  // slots (load <= 1/2)
     // probed with a mask: a size that is not a power of two leaves slots unreachable and the
     // probe loop below would never end
-    static_assert((HC & (HC - 1)) == 0, "kGroupItems x kBlock / G must be a power of two");
+    static_assert((HC & (HC - 1)) == 0, "IT x kBlock / G must be a power of two");
     __shared__ int h_tag[HC];        // (label << 1) | departure, -1: empty
     __shared__ int h_w[HC];
     __shared__ unsigned long long h_key[HC];
@@ -1007,7 +1006,7 @@ __device__ inline int bin_of(const BinConfig& cfg, int deg) {
 }
 
 // counts[l] / edges[l]: nodes and edges of list l = sub-round * kNumBins + bin; with split,
-// counts[kMaxSub * kNumBins + sr] also receives the wave-bin nodes of degree <= kWaveSmall.
+// counts[kMaxSub * kNumBins + sr] also receives the wave-bin nodes of degree <= cfg.wave_small.
 __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, int S,
                           BinConfig cfg, const uint8_t* __restrict__ include, int split,
                           int* __restrict__ counts, unsigned long long* __restrict__ edges) {
@@ -1028,7 +1027,7 @@ __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, in
             const int l = sr * kNumBins + b;
             atomicAdd(&local[l], 1);
             atomicAdd(&ledges[l], (unsigned long long)deg);
-            if (split && b == kNumBins - 2 && deg <= kWaveSmall) atomicAdd(&local[L + sr], 1);
+            if (split && b == kNumBins - 2 && deg <= cfg.wave_small) atomicAdd(&local[L + sr], 1);
         }
     }
     __syncthreads();
@@ -1042,7 +1041,7 @@ __global__ void bin_count(int n, const int* __restrict__ xadj, uint32_t seed, in
         if (local[L + i]) atomicAdd(&counts[L + i], local[L + i]);
 }
 
-// With split, a wave list holds its nodes of degree <= kWaveSmall first (counts[L + sr] of
+// With split, a wave list holds its nodes of degree <= cfg.wave_small first (counts[L + sr] of
 // them, cursor slot L + sr), then the others.
 __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, int S,
                             BinConfig cfg, const uint8_t* __restrict__ include, int split,
@@ -1063,7 +1062,7 @@ __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, 
             l = sr * kNumBins + b;
             slot = l;
             if (split && b == kNumBins - 2) {
-                if (deg <= kWaveSmall) slot = L + sr;  // front part of the wave list
+                if (deg <= cfg.wave_small) slot = L + sr;  // front part of the wave list
                 else skip = counts[L + sr];            // after the small ones
             }
             r = atomicAdd(&local[slot], 1);
@@ -1210,6 +1209,28 @@ __global__ void block_weights_global(int n, const int* __restrict__ part,
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
 
+// The G-lane proposal kernel of one list with `items` nodes per lane group: clustering launches
+// one workgroup per virtual block, refinement strides over its device-side frontier count.
+template <int G>
+void launch_group(bool cl, int items, const LpArgs& a, const int* lst, int cnt, const int* dc,
+                  hipStream_t s) {
+    const int g = grid_for((long long)cnt * G, kBlock * items);
+    const dim3 grid(cl ? g : std::min(g, kFrontierGrid));
+    switch (items) {
+        case 1:
+            if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, G, 1>), grid, dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+            else hipLaunchKernelGGL((propose_group<LpMode::Refine, G, 1>), grid, dim3(kBlock), 0, s, a, lst, cnt, dc);
+            break;
+        case 4:
+            if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, G, 4>), grid, dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+            else hipLaunchKernelGGL((propose_group<LpMode::Refine, G, 4>), grid, dim3(kBlock), 0, s, a, lst, cnt, dc);
+            break;
+        default:
+            if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, G, 2>), grid, dim3(kBlock), 0, s, a, lst, cnt, nullptr);
+            else hipLaunchKernelGGL((propose_group<LpMode::Refine, G, 2>), grid, dim3(kBlock), 0, s, a, lst, cnt, dc);
+    }
+}
+
 }  // namespace
 
 // ================================================================= host side
@@ -1231,6 +1252,11 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
                           const BinConfig& bins, hipStream_t s, const uint8_t* include) {
     KH_REQUIRE(S >= 1 && S <= kMaxSub, "sub-rounds must be in [1, 8]");
     KH_REQUIRE(bins.hi[kNumBins - 2] <= 512, "the wave bin holds degrees <= 512");
+    KH_REQUIRE(bins.group_items == 1 || bins.group_items == 2 || bins.group_items == 4,
+               "group_items must be 1, 2 or 4");
+    KH_REQUIRE(bins.wave_small == 64 || bins.wave_small == 128 || bins.wave_small == 256,
+               "wave_small must be 64, 128 or 256");
+    bins_ = bins;
     KH_REQUIRE(g.n < (1 << 30) && num_labels < (1 << 30), "labels must be below 2^30");
     subrounds_ = S;
     num_labels_ = num_labels;
@@ -1625,26 +1651,19 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 const double bytes = cl ? 12.0 * (double)ledges + 16.0 * lnodes : 0.0;
                 ProfScope ps(prof, base + "_propose_" + kBinName[b], bytes, cl ? ledges : 0, s);
                 switch (b) {
-                    case 0:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 8>), dim3(grid_for((long long)cnt * 8, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 8>), dim3(std::min(grid_for((long long)cnt * 8, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
-                        break;
-                    case 1:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 16>), dim3(grid_for((long long)cnt * 16, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 16>), dim3(std::min(grid_for((long long)cnt * 16, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
-                        break;
-                    case 2:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 32>), dim3(grid_for((long long)cnt * 32, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 32>), dim3(std::min(grid_for((long long)cnt * 32, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
-                        break;
-                    case 3:
-                        if (cl) hipLaunchKernelGGL((propose_group<LpMode::Cluster, 64>), dim3(grid_for((long long)cnt * 64, kBlock * kGroupItems)), dim3(kBlock), 0, s, a, lst, cnt, nullptr);
-                        else hipLaunchKernelGGL((propose_group<LpMode::Refine, 64>), dim3(std::min(grid_for((long long)cnt * 64, kBlock * kGroupItems), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, dc);
-                        break;
+                    case 0: launch_group<8>(cl, bins_.group_items, a, lst, cnt, dc, s); break;
+                    case 1: launch_group<16>(cl, bins_.group_items, a, lst, cnt, dc, s); break;
+                    case 2: launch_group<32>(cl, bins_.group_items, a, lst, cnt, dc, s); break;
+                    case 3: launch_group<64>(cl, bins_.group_items, a, lst, cnt, dc, s); break;
                     case 4:
                         if (cl) {
-                            const int ns = host_wave_small_[sr];  // degree <= kWaveSmall, first
-                            if (ns > 0) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false, 2 * kWaveSmall>), dim3(grid_for(ns, kBlock / 64)), dim3(kBlock), 0, s, a, lst, ns, nl, nullptr);
+                            const int ns = host_wave_small_[sr];  // degree <= wave_small, first
+                            if (ns > 0) {
+                                const dim3 gs(grid_for(ns, kBlock / 64));
+                                if (bins_.wave_small == 64) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false, 128>), gs, dim3(kBlock), 0, s, a, lst, ns, nl, nullptr);
+                                else if (bins_.wave_small == 256) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false, 512>), gs, dim3(kBlock), 0, s, a, lst, ns, nl, nullptr);
+                                else hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false, 256>), gs, dim3(kBlock), 0, s, a, lst, ns, nl, nullptr);
+                            }
                             if (cnt > ns) hipLaunchKernelGGL((propose_wave<LpMode::Cluster, false>), dim3(grid_for(cnt - ns, kBlock / 64)), dim3(kBlock), 0, s, a, lst + ns, cnt - ns, nl, nullptr);
                         }
                         else if (dense_wave_) hipLaunchKernelGGL((propose_wave<LpMode::Refine, true>), dim3(std::min(grid_for(cnt, kBlock / 64), kFrontierGrid)), dim3(kBlock), 0, s, a, lst, cnt, nl, dc);

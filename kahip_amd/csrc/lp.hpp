@@ -79,6 +79,7 @@ private:
     void build_chunk_maps(hipStream_t s);  // cmap_ of every sub-round (after chunk_prefix_)
     int subrounds_ = 1;
     int num_labels_ = 0;
+    BinConfig bins_;  // degree bins and tile shapes of the last prepare()
     bool dense_wave_ = false, dense_hub_ = false;
     // per sub-round hub bookkeeping (hub list = last degree bin)
     int hub_n_[kMaxSub] = {};

@@ -151,14 +151,17 @@ def mirror_partition_plain(xadj, adjncy, vwgt, adjwgt, k, imbalance=0.03, seed=0
 
 
 def mirror_batch(graphs, k, imbalance=0.03, seed=0, cluster_rounds=10, refine_rounds=25,
-                 subrounds=4, cluster_factor=18, stop_factor=5000, ip=None):
+                 subrounds=4, cluster_factor=18, stop_factor=5000, ip=None, ip_tries=-1):
     """CPU mirror of the batched mode: the disjoint union of `graphs` ((xadj, adjncy, vwgt,
     adjwgt) with local ids) under the synchronous schedule with per-label bounds (the cluster
     bound of a node's graph; the block bound U_g of labels g*k + b), per-graph stop rules and
-    per-graph coarsest levels. Returns (local parts per graph, info)."""
+    per-graph coarsest levels; ip_tries: BFS-grown starts per bisection of the initial
+    partitioner (-1: its default). Returns (local parts per graph, info)."""
     if ip is None:
         import kahip_amd
-        ip = kahip_amd.initial_partition
+
+        def ip(*args):
+            return kahip_amd.initial_partition(*args, tries=ip_tries)
     G = len(graphs)
     # the union
     noff = [0]

@@ -85,6 +85,32 @@ def test_generated_batch_balance_and_cuts():
     b.free()
 
 
+@pytest.mark.parametrize("name", ["rmat13", "rgg16"])
+def test_tile_shapes_keep_results(name):
+    """The LP kernels' tile shapes (nodes per lane group, small wave kernel limit) and the wave
+    bin change launches only: the partition is the same bits."""
+    g = K.Graph.rmat(13, 16 << 13, 4) if name == "rmat13" else K.Graph.rgg(1 << 16, 5)
+    want, s0 = g.partition(k=8, seed=1)
+    for items, small, wave_hi in ((1, 64, 96), (4, 256, 512), (1, 256, 256), (4, 64, 128)):
+        part, st = g.partition(k=8, seed=1, group_items=items, wave_small=small,
+                               bins=[8, 16, 32, 64, wave_hi, 0x7fffffff])
+        assert np.array_equal(part, want), (items, small, wave_hi)
+        assert st["cut"] == s0["cut"]
+    g.free()
+
+
+def test_autotune_tile_shapes_keeps_results():
+    g = K.Graph.rmat(13, 16 << 13, 4)
+    tuned, ms = K.autotune(g, k=8)
+    assert len(ms) == K.AUTOTUNE_EVALS and all(t > 0 for t in ms)
+    assert tuned["group_items"] in (1, 2, 4) and tuned["wave_small"] in (64, 128, 256)
+    assert tuned["bins"][:4] == [8, 16, 32, 64] and 96 <= tuned["bins"][4] <= 512
+    p1, _ = g.partition(k=8, seed=1)
+    p2, _ = g.partition(k=8, seed=1, **tuned)
+    assert np.array_equal(p1, p2)
+    g.free()
+
+
 def test_autotune_bins_keeps_results():
     g = K.Graph.rmat(13, 16 << 13, 4)
     bins, ms = K.autotune_bins(g, k=8)

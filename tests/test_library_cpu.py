@@ -137,3 +137,14 @@ def test_bgf_writer_matches_reference_file_and_round_trips(tmp_path, rgg15):
     assert digest == "f5d9828257b35d428d5b8c182e8b58c1c78d51d8b7664a76e0aa164a12cdf81d"
     rx, ra, _, _ = graphs.read_bgf(str(path))
     assert np.array_equal(rx, x) and np.array_equal(ra, a)
+
+
+def test_tile_shape_parameters_are_validated(rgg15):
+    """group_items and wave_small take only the instantiated tile shapes; anything else is an
+    error before any GPU work."""
+    x, a, _, _ = rgg15
+    assert K.default_params().group_items == 2 and K.default_params().wave_small == 128
+    with pytest.raises(K.KahipError, match="group_items"):
+        K.partition(x, a, k=2, group_items=3)
+    with pytest.raises(K.KahipError, match="wave_small"):
+        K.partition(x, a, k=2, wave_small=100)
