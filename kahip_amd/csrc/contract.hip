@@ -15,8 +15,10 @@
 //                 LDS bitonic sort by min e; rows with more distinct neighbours go to
 //   rows_lds_big  one workgroup per row: 256 threads and 32 KB of LDS for <= 1024 distinct, then
 //                 1024 threads and 128 KB of LDS for <= 4096 distinct; beyond that
-//   rows_global_* the row's member edges in chunks of 4096 over many workgroups, LDS
-//                 pre-aggregation, one global table per row, then a segmented radix sort by min e.
+//   rows_global_* the row's members sorted by id, their edges in chunks of 4096 over many
+//                 workgroups: LDS pre-aggregation, a merge per (row, bucket) that stages each
+//                 merged entry at the flattened index of its first edge, and a compaction of the
+//                 staged entries in that index order (= ascending min e, no sort).
 //   compact     rows are gathered into the final CSR after a scan of the coarse degrees.
 
 #include <cstring>
@@ -444,15 +446,20 @@ __global__ void member_degrees(int n, const int* __restrict__ members,
     if (i < n) mdeg[i] = xadj[members[i] + 1] - xadj[members[i]];
 }
 
-// Rows beyond the LDS tiers, split into chunks of kGChunk member edges (the row's member edges
-// flattened through the member-degree prefix mpre). Row i has P_i = min(chunks, kGMaxBuckets)
-// buckets of neighbour clusters q (bucket = gbucket(q, P_i)).
-//   gchunk   each chunk aggregates (q -> sum w, min e) in LDS and writes its entries grouped by
-//            bucket (per-chunk bucket offsets offm, bucket-major counts cntT);
+// Rows beyond the LDS tiers, split into chunks of kGChunk member edges: the row's members are
+// sorted by node id and their edges flattened through the member-degree prefix mpre, so the
+// flattened index f of an edge grows with its CSR index e within a row. Row i has
+// P_i = min(chunks, kGMaxBuckets) buckets of neighbour clusters q (bucket = gbucket(q, P_i)).
+//   gchunk   each chunk aggregates (q -> sum w, min f) in LDS and writes its entries grouped by
+//            bucket (per-chunk bucket offsets offm, bucket-major counts cntT); it also clears
+//            the stage slots of its f range;
 //   gscatter after a scan of cntT, moves them so that every (row, bucket) is one range;
-//   gmerge   one workgroup per (row, bucket) sums the range in LDS and appends the merged
-//            entries (key min e, value (q, w)) to the row's output run; ranges of more than
-//            kGMergeMax entries are queued for gmerge_global (a private global table per task).
+//   gmerge   one workgroup per (row, bucket) sums the range in LDS and stores every merged
+//            entry (q, w) in the stage at f = its min f; ranges of more than kGMergeMax entries
+//            are queued for gmerge_global (a private global table per task);
+//   gemit    counts the staged entries of each chunk, then (after a scan over the chunks)
+//            writes them in f order, which is the order of the first appearance of q in the
+//            reference's scan of the row (complete_boundary.h:511-562).
 // No global atomics on shared tables: every merge is local to one workgroup.
 constexpr int kGMaxBuckets = 1024;
 constexpr int kGMergeCap = 8192;  // LDS slots of gmerge
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
     const int* __restrict__ mpre, const long long* __restrict__ offm_base, int* __restrict__ offm,
     const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials,
-    int* __restrict__ err) {
+    int* __restrict__ stage_q, int* __restrict__ err) {
     __shared__ int K[kGCap];
     __shared__ int V[kGCap];
     __shared__ int M[kGCap];
@@ -527,6 +534,7 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     __syncthreads();
     const int m0 = m0s, nm = nms;
     const bool staged = nm <= kGChunk;  // uniform
+    for (long long f = f0 + threadIdx.x; f < f1; f += kGThreads) stage_q[f] = -1;
     if (staged)
         for (int i = threadIdx.x; i < nm; i += kGThreads) {
             sbeg[i] = mpre[m0 + i];
@@ -573,7 +581,7 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
             continue;
         }
         atomicAdd(&V[h], ws[j]);
-        atomicMin(&M[h], es[j]);
+        atomicMin(&M[h], (int)(f0 + threadIdx.x + (long long)j * kGThreads));  // this edge's f
     }
     for (int i = threadIdx.x; i <= P; i += kGThreads) bcur[i] = 0;
     __syncthreads();
@@ -647,19 +655,15 @@ __global__ __launch_bounds__(256) void gscatter(
 __global__ __launch_bounds__(512) void gmerge(
     const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix, int count,
     int n_tasks, const long long* __restrict__ ct_base, const int* __restrict__ posT,
-    const int4* __restrict__ bucketed, const int* __restrict__ obase, int* __restrict__ rowcnt,
-    unsigned* __restrict__ okey, unsigned long long* __restrict__ oval,
+    const int4* __restrict__ bucketed, int* __restrict__ stage_q, int* __restrict__ stage_w,
     int* __restrict__ big_tasks, int* __restrict__ n_big, int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
-    __shared__ int row, nout, base;
+    __shared__ int row;
     const int task = blockIdx.x;
     if (task >= n_tasks) return;  // uniform
-    if (threadIdx.x == 0) {
-        row = row_of_chunk(task_prefix, count, task);
-        nout = 0;
-    }
+    if (threadIdx.x == 0) row = row_of_chunk(task_prefix, count, task);
     __syncthreads();
     const int ri = row;
     const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
@@ -690,18 +694,11 @@ __global__ __launch_bounds__(512) void gmerge(
         atomicMin(&M[h], e.z);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < cap; i += 512)
-        if (K[i] >= 0) atomicAdd(&nout, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) base = obase[ri] + atomicAdd(&rowcnt[ri], nout);
-    if (threadIdx.x == 0) nout = 0;
-    __syncthreads();
     for (int i = threadIdx.x; i < cap; i += 512) {
         const int q = K[i];
         if (q < 0) continue;
-        const int at = base + atomicAdd(&nout, 1);
-        okey[at] = (unsigned)M[i];
-        oval[at] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[i];
+        stage_q[M[i]] = q;  // one merged entry per (row, q): its min f is unique
+        stage_w[M[i]] = V[i];
     }
 }
 
@@ -712,17 +709,13 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     const int* __restrict__ task_prefix, int count, const long long* __restrict__ ct_base,
     const int* __restrict__ posT, const int4* __restrict__ bucketed,
     const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
-    int* __restrict__ TV, int* __restrict__ TM, const int* __restrict__ obase,
-    int* __restrict__ rowcnt, unsigned* __restrict__ okey, unsigned long long* __restrict__ oval,
-    int* __restrict__ err) {
-    __shared__ int row, nout, base;
+    int* __restrict__ TV, int* __restrict__ TM, int* __restrict__ stage_q,
+    int* __restrict__ stage_w, int* __restrict__ err) {
+    __shared__ int row;
     const int t = blockIdx.x;
     if (t >= n_big) return;
     const int task = big_tasks[t];
-    if (threadIdx.x == 0) {
-        row = row_of_chunk(task_prefix, count, task);
-        nout = 0;
-    }
+    if (threadIdx.x == 0) row = row_of_chunk(task_prefix, count, task);
     __syncthreads();
     const int ri = row;
     const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
@@ -745,25 +738,12 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     }
     __threadfence();  // the table's atomics complete before it is read back (rare path)
     __syncthreads();
-    for (long long i = threadIdx.x; i < cap; i += 1024)
-        if (K[i] >= 0) atomicAdd(&nout, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) base = obase[ri] + atomicAdd(&rowcnt[ri], nout);
-    if (threadIdx.x == 0) nout = 0;
-    __syncthreads();
     for (long long i = threadIdx.x; i < cap; i += 1024) {
         const int q = K[i];
         if (q < 0) continue;
-        const int at = base + atomicAdd(&nout, 1);
-        okey[at] = (unsigned)Mn[i];
-        oval[at] = ((unsigned long long)(uint32_t)q << 32) | (uint32_t)V[i];
+        stage_q[Mn[i]] = q;
+        stage_w[Mn[i]] = V[i];
     }
-}
-
-__global__ void row_counts_to_cdeg(const int* __restrict__ list, int count,
-                                   const int* __restrict__ rowcnt, int* __restrict__ cdeg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) cdeg[list[i]] = rowcnt[i];
 }
 
 __global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
@@ -780,27 +760,98 @@ __global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
     totals[t] = posT[ct_base[ri] + (long long)(b + 1) * nch] - posT[ct_base[ri] + (long long)b * nch];
 }
 
-__global__ void overflow_ends(const int* __restrict__ list, int count, const int* __restrict__ obase,
-                              const int* __restrict__ cdeg, int* __restrict__ oend) {
+__global__ void member_segments(const int* __restrict__ list, int count,
+                                const int* __restrict__ moff, int* __restrict__ beg,
+                                int* __restrict__ end) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) oend[i] = obase[i] + cdeg[list[i]];
+    if (i >= count) return;
+    beg[i] = moff[list[i]];
+    end[i] = moff[list[i] + 1];
 }
 
-__global__ void overflow_copyback(const int* __restrict__ list, int count,
-                                  const int* __restrict__ obase, const int* __restrict__ cdeg,
-                                  const int* __restrict__ tpos,
-                                  const unsigned long long* __restrict__ oval_sorted,
-                                  int* __restrict__ tq, int* __restrict__ tw) {
-    const int i = blockIdx.x;
-    if (i >= count) return;
-    const int p = list[i];
-    const int c = cdeg[p];
-    const int src = obase[i], dst = tpos[p];
-    for (int j = threadIdx.x; j < c; j += blockDim.x) {
-        const unsigned long long x = oval_sorted[src + j];
-        tq[dst + j] = (int)(x >> 32);
-        tw[dst + j] = (int)(x & 0xffffffffu);
+// The f range [f0, f1) of global chunk `chunk` (row ri of the list).
+__device__ inline void chunk_range(const int* __restrict__ list, const int* __restrict__ chunk_prefix,
+                                   const int* __restrict__ moff, const int* __restrict__ mpre,
+                                   int ri, int chunk, long long& f0, long long& f1) {
+    const int p = list[ri];
+    const long long fr0 = mpre[moff[p]], fr1 = mpre[moff[p + 1]];
+    f0 = fr0 + (long long)(chunk - chunk_prefix[ri]) * kGChunk;
+    f1 = min(fr1, f0 + kGChunk);
+}
+
+constexpr int kEmitThreads = 256;  // kGChunk / kEmitThreads = 16 staged slots per thread
+static_assert(kGChunk % kEmitThreads == 0, "every emission thread takes whole slots");
+
+// Staged entries of each chunk (one workgroup per chunk).
+__global__ __launch_bounds__(kEmitThreads) void gemit_count(
+    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    const int* __restrict__ moff, const int* __restrict__ mpre, const int* __restrict__ stage_q,
+    int* __restrict__ ecnt) {
+    __shared__ int row, wsum[kEmitThreads / 64];
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
+    __syncthreads();
+    long long f0, f1;
+    chunk_range(list, chunk_prefix, moff, mpre, row, chunk, f0, f1);
+    int c = 0;
+    for (long long f = f0 + threadIdx.x; f < f1; f += kEmitThreads) c += stage_q[f] >= 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kEmitThreads / 64; ++w) t += wsum[w];
+        ecnt[chunk] = t;
     }
+}
+
+// Writes each chunk's staged entries in f order at tpos[p] + (entries of the row's earlier
+// chunks) + (rank in the chunk); the row's last chunk also writes its coarse degree. Thread t
+// takes the chunk's slots [16 t, 16 t + 16), so ranks follow f.
+__global__ __launch_bounds__(kEmitThreads) void gemit_write(
+    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
+    const int* __restrict__ moff, const int* __restrict__ mpre, const int* __restrict__ stage_q,
+    const int* __restrict__ stage_w, const int* __restrict__ epos, const int* __restrict__ tpos,
+    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw) {
+    constexpr int PER = kGChunk / kEmitThreads;
+    __shared__ int row, wsum[kEmitThreads / 64];
+    const int chunk = blockIdx.x;
+    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
+    __syncthreads();
+    const int ri = row;
+    const int p = list[ri];
+    long long f0, f1;
+    chunk_range(list, chunk_prefix, moff, mpre, ri, chunk, f0, f1);
+    const long long a0 = f0 + (long long)threadIdx.x * PER;
+    int qs[PER];
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        qs[j] = a0 + j < f1 ? stage_q[a0 + j] : -1;
+        mine += qs[j] >= 0;
+    }
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if ((int)(threadIdx.x & 63) >= o) incl += y;
+    }
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int before = incl - mine;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wsum[w];
+    const int first = chunk_prefix[ri];
+    int at = tpos[p] + (epos[chunk] - epos[first]) + before;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        if (qs[j] < 0) continue;
+        tq[at] = qs[j];
+        tw[at] = stage_w[a0 + j];
+        ++at;
+    }
+    if (threadIdx.x == 0 && chunk + 1 == chunk_prefix[ri + 1])
+        cdeg[p] = epos[chunk + 1] - epos[first];
 }
 
 __global__ void compact_rows(int nc, const int* __restrict__ tpos, const int* __restrict__ cxadj,
@@ -822,18 +873,10 @@ __global__ void project_kernel(int n, const int* __restrict__ map,
     if (v < n) fine_part[v] = coarse_part[map[v]];
 }
 
-__global__ void hub_cap_kernel(const int* __restrict__ list, int count, const int* __restrict__ fdeg,
-                               int nc, long long* __restrict__ cap, int* __restrict__ bound,
-                               int* __restrict__ fd_out) {
+__global__ void row_fdeg(const int* __restrict__ list, int count, const int* __restrict__ fdeg,
+                         int* __restrict__ fd_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int fd = fdeg[list[i]];
-    fd_out[i] = fd;
-    const long long distinct = fd < nc ? fd : nc;
-    long long c = 64;
-    while (c < 2 * distinct) c <<= 1;
-    cap[i] = c;
-    bound[i] = (int)distinct;
+    if (i < count) fd_out[i] = fdeg[list[i]];
 }
 
 }  // namespace
@@ -951,27 +994,18 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     KH_CHECK(hipStreamSynchronize(s));
     if (novl > 0) {
         ProfScope ps(prof, "contract_rows_global", 0.0, novl, s);
-        hcap_.reserve(novl);
-        hoff_.reserve(novl);
-        obound_.reserve(novl + 1);
-        obase_.reserve(novl + 1);
-        oend_.reserve(novl);
         ofd_.reserve(novl);
-        hipLaunchKernelGGL(hub_cap_kernel, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl2_list_.p, novl, fdeg_.p, nc, hcap_.p, obound_.p, ofd_.p);
-        std::vector<int> bounds(novl), bases(novl), fd(novl);
-        KH_CHECK(hipMemcpyAsync(bounds.data(), obound_.p, sizeof(int) * novl,
-                                hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(row_fdeg, dim3(grid_for(novl, 256)), dim3(256), 0, s, ovl2_list_.p,
+                           novl, fdeg_.p, ofd_.p);
+        std::vector<int> fd(novl);
         KH_CHECK(hipMemcpyAsync(fd.data(), ofd_.p, sizeof(int) * novl, hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
         // per row: chunks of its member edges, buckets, count / offset table bases, tasks
         std::vector<int> cpre(novl + 1), tpre(novl + 1);
         std::vector<long long> ofb(novl), ctb(novl);
-        long long ototal = 0, offm_total = 0, ct_total = 0;
+        long long offm_total = 0, ct_total = 0;
         int nchunks = 0, ntasks = 0;
         for (int i = 0; i < novl; ++i) {
-            bases[i] = (int)ototal;
-            ototal += bounds[i];
             const int nch = std::max(1, (fd[i] + kGChunk - 1) / kGChunk);
             const int P = std::min(nch, kGMaxBuckets);
             cpre[i] = nchunks;
@@ -985,14 +1019,11 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         }
         cpre[novl] = nchunks;
         tpre[novl] = ntasks;
-        KH_REQUIRE(ototal < (1LL << 31) && ct_total < (1LL << 31),
-                   "contraction scratch exceeds 2^31 entries");
-        obase_.reserve(novl + 1);
+        KH_REQUIRE(ct_total < (1LL << 31), "contraction scratch exceeds 2^31 entries");
         gchunk_prefix_.reserve(novl + 1);
         gtask_prefix_.reserve(novl + 1);
         goffm_base_.reserve(novl);
         gct_base_.reserve(novl);
-        KH_CHECK(hipMemcpyAsync(obase_.p, bases.data(), sizeof(int) * novl, hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(gchunk_prefix_.p, cpre.data(), sizeof(int) * (novl + 1),
                                 hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(gtask_prefix_.p, tpre.data(), sizeof(int) * (novl + 1),
@@ -1001,37 +1032,54 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
                                 hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(gct_base_.p, ctb.data(), sizeof(long long) * novl,
                                 hipMemcpyHostToDevice, s));
-        okey_.reserve(ototal);
-        oval_.reserve(ototal);
-        okey2_.reserve(ototal);
-        oval2_.reserve(ototal);
         goffm_.reserve(offm_total);
         gcnt_.reserve(ct_total + 1);
         gpos_.reserve(ct_total + 1);
         gpart_.reserve((long long)nchunks * kGChunk);
         gbuck_.reserve((long long)nchunks * kGChunk);
-        growcnt_.reserve(novl);
         gbig_.reserve(std::max(ntasks, 1));
         gnbig_.reserve(1);
         fill_async(gcnt_.p + ct_total, 1, 0, s);  // sentinel
-        fill_async(growcnt_.p, novl, 0, s);
         fill_async(gnbig_.p, 1, 0, s);
+        // the members of the global rows in ascending node id (the other rows keep theirs), so
+        // that a row's flattened edge index f grows with the CSR index e
+        members_s_.reserve(std::max(n, 1));
+        mseg_.reserve(2LL * novl);
+        if (n > 0)
+            KH_CHECK(hipMemcpyAsync(members_s_.p, members_.p, sizeof(int) * n,
+                                    hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(member_segments, dim3(grid_for(novl, 256)), dim3(256), 0, s,
+                           ovl2_list_.p, novl, moff_.p, mseg_.p, mseg_.p + novl);
+        {
+            int bits = 1;
+            while (bits < 31 && (1LL << bits) < (long long)n) ++bits;
+            size_t bytes = 0;
+            KH_CHECK(rocprim::segmented_radix_sort_keys(nullptr, bytes, members_.p,
+                                                        members_s_.p, (unsigned)n, (unsigned)novl,
+                                                        mseg_.p, mseg_.p + novl, 0, bits, s));
+            char* t = scan_tmp_.reserve((long long)bytes + 1);
+            KH_CHECK(rocprim::segmented_radix_sort_keys(t, bytes, members_.p, members_s_.p,
+                                                        (unsigned)n, (unsigned)novl, mseg_.p,
+                                                        mseg_.p + novl, 0, bits, s));
+        }
         mdeg_.reserve((long long)n + 1);
         mpre_.reserve((long long)n + 1);
-        hipLaunchKernelGGL(member_degrees, dim3(grid_for(n, 256)), dim3(256), 0, s, n, members_.p,
-                           g.xadj, mdeg_.p);
+        hipLaunchKernelGGL(member_degrees, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                           members_s_.p, g.xadj, mdeg_.p);
         fill_async(mdeg_.p + n, 1, 0, s);
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
+        stage_q_.reserve(std::max<long long>(m2, 1));
+        stage_w_.reserve(std::max<long long>(m2, 1));
         hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
-                           gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
-                           mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p, gpart_.p,
-                           err_.p);
+                           gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
+                           members_s_.p, mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p,
+                           gpart_.p, stage_q_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
         hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,
                            goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
         hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(512), 0, s, gchunk_prefix_.p,
-                           gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p, obase_.p,
-                           growcnt_.p, okey_.p, oval_.p, gbig_.p, gnbig_.p, err_.p);
+                           gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p,
+                           stage_q_.p, stage_w_.p, gbig_.p, gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -1069,25 +1117,20 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             fill_async(hm_.p, ttotal, 0x7fffffff, s);
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
                                gchunk_prefix_.p, gtask_prefix_.p, novl, gct_base_.p, gpos_.p,
-                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, obase_.p,
-                               growcnt_.p, okey_.p, oval_.p, err_.p);
+                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, stage_q_.p,
+                               stage_w_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
         }
-        hipLaunchKernelGGL(row_counts_to_cdeg, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl2_list_.p, novl, growcnt_.p, cdeg_.p);
-        hipLaunchKernelGGL(overflow_ends, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl2_list_.p, novl, obase_.p, cdeg_.p, oend_.p);
-        size_t bytes = 0;
-        KH_CHECK(rocprim::segmented_radix_sort_pairs(
-            nullptr, bytes, okey_.p, okey2_.p, oval_.p, oval2_.p, (unsigned)ototal,
-            (unsigned)novl, obase_.p, oend_.p, 0, 32, s));
-        char* t = scan_tmp_.reserve((long long)bytes + 1);
-        KH_CHECK(rocprim::segmented_radix_sort_pairs(t, bytes, okey_.p, okey2_.p, oval_.p,
-                                                     oval2_.p, (unsigned)ototal, (unsigned)novl,
-                                                     obase_.p, oend_.p, 0, 32, s));
-        hipLaunchKernelGGL(overflow_copyback, dim3(novl), dim3(256), 0, s, ovl2_list_.p, novl,
-                           obase_.p, cdeg_.p, tpos_.p, oval2_.p, tq_.p, tw_.p);
-        KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
+        // emission in f order: counts per chunk, a scan over the chunks, ordered writes
+        ecnt_.reserve((long long)nchunks + 1);
+        epos_.reserve((long long)nchunks + 1);
+        fill_async(ecnt_.p + nchunks, 1, 0, s);
+        hipLaunchKernelGGL(gemit_count, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
+                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, stage_q_.p, ecnt_.p);
+        scan_i32(ecnt_.p, epos_.p, nchunks + 1, s);
+        hipLaunchKernelGGL(gemit_write, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
+                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, stage_q_.p, stage_w_.p,
+                           epos_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
     }
     scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
     int mc = 0, err = 0;

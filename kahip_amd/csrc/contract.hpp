@@ -37,16 +37,17 @@ private:
     DevBuf<char> scan_tmp_;
     DevBuf<int> leader_, flag_, newid_;
     DevBuf<int> cnt_, fdeg_, moff_, tpos_, cursor_, cdeg_, members_, tq_, tw_;
-    DevBuf<int> ovl_list_, ovl2_list_, ovl3_list_, ovl_count_, obound_, obase_, oend_;
+    DevBuf<int> ovl_list_, ovl2_list_, ovl3_list_, ovl_count_;
     DevBuf<int> rsmall_, rmid_, rcount_;
     DevBuf<long long> hcap_, hoff_;
     DevBuf<int> gchunk_prefix_, mdeg_, mpre_, ofd_;
     DevBuf<int> hk_, hv_, hm_;
-    DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, growcnt_, gbig_, gnbig_;
+    DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, gbig_, gnbig_;
     DevBuf<long long> goffm_base_, gct_base_;
     DevBuf<int4> gpart_, gbuck_;
-    DevBuf<unsigned> okey_, okey2_;
-    DevBuf<unsigned long long> oval_, oval2_;
+    DevBuf<int> members_s_, mseg_;           // global rows' members sorted; segment bounds
+    DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
+    DevBuf<int> ecnt_, epos_;                // staged entries per chunk and their scan
     DevBuf<int> err_;  // device error flag (kErrProbe) of the last contract()
 };
 
