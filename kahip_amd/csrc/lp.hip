@@ -180,6 +180,50 @@ __device__ inline void decide(const LpArgs& a, int v, int own, int w, int conn_o
     add_contrib(a, propose<M>(a, v, own, w, conn_own, c));
 }
 
+// Decision scan of a wave's LDS label table: the best eligible label among the slots lane,
+// lane + 64, ... below cap (keys K, summed weights V; empty slots -1) and the weight towards the
+// own label. A pass takes SLOTS slots per lane: it reads their keys and values, then issues the
+// label-weight (and per-label bound) gathers of all of them, then evaluates, so a lane waits for
+// one gather latency per pass instead of one per slot. Candidates compare by the total order of
+// better(), so the result does not depend on the scan order. (The workgroup-per-hub kernels keep
+// the slot-by-slot scan: there the batched form measured slower.)
+template <int SLOTS>
+__device__ __attribute__((always_inline)) inline void scan_wave_table(
+    const LpArgs& a, const int* K, const int* V, int cap, int lane, int v, int own, int nw,
+    int& conn_own, Cand& c) {
+    for (int base = lane; base < cap; base += SLOTS * 64) {
+        int lk[SLOTS], lv[SLOTS], ls[SLOTS], lb[SLOTS];
+#pragma unroll
+        for (int j = 0; j < SLOTS; ++j) {
+            const int i = base + j * 64;
+            lk[j] = i < cap ? K[i] : -1;
+            lv[j] = lk[j] >= 0 ? V[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < SLOTS; ++j) ls[j] = lk[j] >= 0 && lk[j] != own ? a.size[lk[j]] : 0;
+        if (a.lbound) {  // uniform (a select between the two would put a.bound in scratch)
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j)
+                lb[j] = lk[j] >= 0 && lk[j] != own ? a.lbound[lk[j]] : 0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j) lb[j] = a.bound;
+        }
+#pragma unroll
+        for (int j = 0; j < SLOTS; ++j) {
+            const int lab = lk[j];
+            if (lab < 0) continue;
+            if (lab == own) {
+                conn_own = lv[j];
+                continue;
+            }
+            if ((long long)ls[j] + nw >= (long long)lb[j] || lv[j] < c.conn) continue;
+            const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+            if (better(lv[j], t, lab, c.conn, c.tie, c.lab)) c = Cand{lv[j], t, lab, ls[j]};
+        }
+    }
+}
+
 // ---------------------------------------------------------------- G lanes per node (deg <= G)
 // dcount: the list's length on the device (the compacted frontier of a refinement round);
 // workgroups then stride over the list's virtual blocks, so a small frontier costs few of them.
@@ -395,9 +439,12 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     const int nw = valid ? a.vwgt[v] : 0;
     int conn_own = 0;
     Cand c{-1, 0u, 0x7fffffff, 0};
-    if (valid) {
+    if (valid && !DENSE) {
+        constexpr int SL = WCAP / 64 < 8 ? WCAP / 64 : 8;
+        scan_wave_table<SL>(a, K, V, cap, lane, v, own, nw, conn_own, c);
+    } else if (valid) {
         for (int i = lane; i < cap; i += 64) {
-            const int lab = DENSE ? (V[i] ? i : -1) : K[i];
+            const int lab = V[i] ? i : -1;
             if (lab < 0) continue;
             const int conn = V[i];
             if (lab == own) {
