@@ -6,26 +6,28 @@
 // The reference scans the members of coarse node p in ascending id and their edges in CSR order
 // and emits each neighbouring cluster q at its first appearance. Since the members' CSR rows are
 // contiguous and increasing in node id, that order is the order of min{e : src(e) in p,
-// cluster(adjncy[e]) = q}. The kernels aggregate (sum of weights, min fine edge index) per (p, q)
-// in any thread order and then sort each row by the min edge index: bit-exact by construction.
+// cluster(adjncy[e]) = q}. The kernels aggregate (sum of weights, first appearance) per (p, q) in
+// any thread order and emit each row in first-appearance order: bit-exact by construction.
 //
 //   remap       leader[c] = atomicMin over members; flags of leaders; exclusive scan -> new ids.
-//   count       coarse weights, member counts, summed member degrees (row capacity bound).
-//   rows_lds      one wave per coarse node: LDS hash (q -> w, min e) of <= 512 entries, then an
-//                 LDS bitonic sort by min e; rows with more distinct neighbours go to
-//   rows_lds_big  one workgroup per row: 256 threads and 32 KB of LDS for <= 1024 distinct, then
-//                 1024 threads and 128 KB of LDS for <= 4096 distinct; beyond that
-//   rows_global_* the row's members sorted by id, their edges in chunks of 4096 over many
-//                 workgroups: LDS pre-aggregation, a merge per (row, bucket) that stages each
-//                 merged entry at the flattened index of its first edge, and a compaction of the
-//                 staged entries in that index order (= ascending min e, no sort).
+//   count       coarse weights, member counts, summed member degrees fd (row capacity bound);
+//               the members of every coarse node in ascending id (a stable radix sort by
+//               cluster), so that the row's edge t = 0, 1, ... grows with the fine edge index.
+//   rows_wave   one wave per row of fd <= 128 / <= 512: LDS hash (q -> w, min t), a stage
+//               indexed by t that marks each entry at its first appearance, one ordered pass;
+//   rows_block  the same with one workgroup per row of fd <= 1024 (256 threads) / <= 4096 (1024);
+//   rows_global_* rows of larger fd: their edges in chunks of 4096 over many workgroups: LDS
+//               pre-aggregation, a merge per (row, bucket) that stages each merged entry at the
+//               flattened index of its first edge, and a compaction of the staged entries in
+//               that index order (= ascending min e, no sort).
 //   compact     rows are gathered into the final CSR after a scan of the coarse degrees.
+// Rows are binned by fd, an upper bound of their distinct neighbours, so no tier overflows.
 
 #include <cstring>
 #include <vector>
 
 #include <rocprim/device/device_scan.hpp>
-#include <rocprim/device/device_segmented_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.hpp"
 #include "contract.hpp"
@@ -35,29 +37,21 @@ namespace kahip {
 
 namespace {
 
-constexpr int kRowCap = 1024;     // LDS hash slots per row (one wave per workgroup)
-constexpr int kRowSmall = 128;    // rows with summed member degree <= this use 256 slots
-constexpr int kDirectGlobal = 4096;  // rows with more member edges go to the chunked tier
-constexpr int kBigThreads = 1024;  // second tier: one large workgroup per row
-constexpr int kBigCap = 8192;      // LDS hash slots (12 B each)
-constexpr int kBigMaxDistinct = 4096;
-constexpr int kMidThreads = 256;   // between them: one 256-thread workgroup per row,
-constexpr int kMidMaxDistinct = 1024;  // 24 KB of hash and 8 KB of sort keys (5 per CU)
+constexpr int kWaveTierCap[2] = {2 * 128, 2 * 512};  // LDS slots of the one-wave tiers
+constexpr int kMidThreads = 256;     // one-workgroup tiers: rows of fd <= 1024 (24 KB of hash,
+constexpr int kMidCap = 2 * 1024;    // 8 KB of member staging, 5 per CU) ...
+constexpr int kBigThreads = 1024;    // ... and of fd <= 4096 (96 KB + 32 KB)
+constexpr int kBigCap = 2 * 4096;
+constexpr int kTierFd[4] = {128, 512, 1024, 4096};  // largest fd of each LDS tier
+static_assert(2 * kTierFd[0] == kWaveTierCap[0] && 2 * kTierFd[1] == kWaveTierCap[1] &&
+                  2 * kTierFd[2] == kMidCap && 2 * kTierFd[3] == kBigCap,
+              "every LDS tier holds rows of up to half its table in member edges");
 constexpr int kGChunk = 4096;      // member edges per workgroup in the global tier
 constexpr int kGCap = 8192;        // LDS hash slots of a global-tier chunk
 constexpr int kGThreads = 512;
 static_assert(2 * kGChunk <= kGCap, "a global-tier chunk fills at most half its table");
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
-
-// The row kernels' early-exit flag lives in LDS and is polled between insertions without a
-// barrier: workgroup-scope atomic loads and stores keep the compiler from caching it.
-__device__ inline int lds_flag(const int* f) {
-    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ inline void lds_raise(int* f) {
-    __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 __global__ void leader_min(int n, const int* __restrict__ label, int* __restrict__ leader) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,88 +81,81 @@ __global__ void count_members(int n, const int* __restrict__ xadj, const int* __
     atomicAdd(&cvwgt[p], vwgt[v]);
 }
 
-__global__ void scatter_members(int n, const int* __restrict__ map, const int* __restrict__ moff,
-                                int* __restrict__ cursor, int* __restrict__ members) {
+__global__ void iota_kernel(int n, int* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const int p = map[v];
-    members[moff[p] + atomicAdd(&cursor[p], 1)] = v;
+    if (v < n) out[v] = v;
 }
 
-// One wave (= one workgroup) per coarse node.
-// Rows of coarse nodes by summed member degree: fdeg <= kRowSmall go to the small-LDS variant
-// (it cannot overflow), the rest to the kRowCap variant. Workgroup-aggregated list appends.
+// Rows by summed member degree fd (an upper bound of the row's distinct neighbours), so that no
+// tier can overflow: fd <= 128 and fd <= 512 go to the one-wave tiers, fd <= 1024 and fd <= 4096
+// to the one-workgroup tiers, larger rows to the chunked global tier. Workgroup-aggregated list
+// appends.
 __global__ __launch_bounds__(256) void row_bins(int nc, const int* __restrict__ fdeg,
-                                                int* __restrict__ small, int* __restrict__ mid,
+                                                int* __restrict__ l0, int* __restrict__ l1,
+                                                int* __restrict__ l2, int* __restrict__ l3,
                                                 int* __restrict__ counts,
                                                 int* __restrict__ global_list,
                                                 int* __restrict__ global_count) {
-    __shared__ int ls, lm, lg, bs, bm, bg;
-    if (threadIdx.x == 0) ls = lm = lg = 0;
+    __shared__ int lc[5], bc[5];
+    if (threadIdx.x < 5) lc[threadIdx.x] = 0;
     __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1, rank = 0;
     if (p < nc) {
         const int fd = fdeg[p];
-        cls = fd <= kRowSmall ? 0 : (fd <= kDirectGlobal ? 1 : 2);
-        rank = atomicAdd(cls == 0 ? &ls : (cls == 1 ? &lm : &lg), 1);
+        cls = fd <= kTierFd[0] ? 0 : fd <= kTierFd[1] ? 1 : fd <= kTierFd[2] ? 2 : fd <= kTierFd[3] ? 3 : 4;
+        rank = atomicAdd(&lc[cls], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        bs = ls ? atomicAdd(&counts[0], ls) : 0;
-        bm = lm ? atomicAdd(&counts[1], lm) : 0;
-        bg = lg ? atomicAdd(global_count, lg) : 0;
+    if (threadIdx.x < 5) {
+        const int c = lc[threadIdx.x];
+        bc[threadIdx.x] = c ? atomicAdd(threadIdx.x < 4 ? &counts[threadIdx.x] : global_count, c) : 0;
     }
     __syncthreads();
-    if (cls == 0) small[bs + rank] = p;
-    else if (cls == 1) mid[bm + rank] = p;
-    else if (cls == 2) global_list[bg + rank] = p;
+    if (cls < 0) return;
+    int* const lists[5] = {l0, l1, l2, l3, global_list};
+    lists[cls][bc[cls] + rank] = p;
 }
 
-// One wave per coarse row. The members' edges are expanded with a load-balanced scan (64
-// members at a time, each lane takes every 64th edge of the batch), their targets' clusters are
-// gathered kEpl at a time, and (q -> sum w, min e) is aggregated in an LDS hash; then the
-// entries are sorted by min e (LDS bitonic sort) and written at the row's capacity offset. With
-// CHECK, more than CAP / 2 distinct entries abandon the row to the overflow list.
-template <int CAP, bool CHECK>
-__global__ __launch_bounds__(64) void rows_lds(
+// The two row kernels below aggregate a coarse row p in LDS: its members (ascending node id)
+// are expanded with a load-balanced scan, and edge t of the row (t = 0, 1, ... over the members'
+// CSR rows in order, so t grows with the fine edge index e) adds (q -> sum w, min t) to an LDS
+// hash, q = cluster of the target (q != p). A row of fd member edges has at most fd distinct
+// entries, and every tier takes only rows whose fd fits half its table. Then stage[min t] = slot
+// marks each entry at its first appearance, and one ordered pass over t writes the row in the
+// reference's order (complete_boundary.h:511-562) at its capacity offset tpos[p].
+
+// One wave (= one workgroup) per row of fd <= CAP / 2.
+template <int CAP>
+__global__ __launch_bounds__(64) void rows_wave(
     const int* __restrict__ rows, int nrows, const int* __restrict__ xadj,
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
     const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
-    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count,
-    int* __restrict__ err) {
+    int* __restrict__ tw, int* __restrict__ err) {
     constexpr int MAXD = CAP / 2;
     constexpr int kEpl = 4;
-    // with CHECK, inserts that have not yet seen `over` add at most 64 x kEpl entries past MAXD
-    static_assert(!CHECK || MAXD + 64 * kEpl <= CAP, "rows_lds table too small");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
     __shared__ int M[CAP];
-    __shared__ unsigned long long S[MAXD];
+    __shared__ int stage[MAXD];
     __shared__ int mpre[64];
     __shared__ int mstart[64];
-    __shared__ int used;
-    __shared__ int over;
-    __shared__ int nkeys;
     if ((int)blockIdx.x >= nrows) return;  // uniform
     const int p = rows[blockIdx.x];
     const int lane = threadIdx.x;
-    const int fd = fdeg[p];
+    const int fd = fdeg[p];  // <= MAXD (row_bins)
     int cap = 64;
-    while (cap < 2 * fd && cap < CAP) cap <<= 1;
+    while (cap < 2 * fd) cap <<= 1;
     for (int i = lane; i < cap; i += 64) {
         K[i] = -1;
         V[i] = 0;
         M[i] = 0x7fffffff;
     }
-    if (lane == 0) {
-        used = 0;
-        over = 0;
-        nkeys = 0;
-    }
+    for (int t = lane; t < fd; t += 64) stage[t] = -1;
     __syncthreads();
     const int mb = moff[p], me = moff[p + 1];
+    int tbase = 0;
     for (int base = mb; base < me; base += 64) {
         const int mi = base + lane;
         int b = 0, d = 0;
@@ -214,109 +201,74 @@ __global__ __launch_bounds__(64) void rows_lds(
             for (int j = 0; j < kEpl; ++j) {
                 const int q = qs[j];
                 if (es[j] < 0 || q == p) continue;
-                if (CHECK && lds_flag(&over)) break;
-                // distinct entries: <= fdeg <= CAP / 2 without CHECK; with CHECK at most
-                // MAXD + 64 x kEpl before every lane has seen `over`
-                bool fresh;
-                const int h = probe_claim(K, cap, q, fresh);
+                const int h = probe_claim(K, cap, q);  // distinct <= fd <= cap / 2
                 if (h < 0) {
                     *err = kErrProbe;
                     continue;
                 }
-                if (CHECK && fresh && atomicAdd(&used, 1) >= MAXD) lds_raise(&over);
                 atomicAdd(&V[h], ws[j]);
-                atomicMin(&M[h], es[j]);
+                atomicMin(&M[h], tbase + t0 + lane + 64 * j);
             }
         }
-        __syncthreads();
-        if (CHECK && over) break;  // uniform after the barrier
+        tbase += total;
+        __syncthreads();  // mpre / mstart are rewritten by the next batch
     }
-    if (CHECK && over) {
-        if (lane == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
-        return;
-    }
-    // compact (min e, slot) keys, pad to a power of two, bitonic sort ascending
-    for (int i = lane; i < cap; i += 64) {
-        if (K[i] >= 0) {
-            const int idx = atomicAdd(&nkeys, 1);
-            S[idx] = ((unsigned long long)(uint32_t)M[i] << 32) | (uint32_t)i;
-        }
-    }
+    for (int i = lane; i < cap; i += 64)
+        if (K[i] >= 0) stage[M[i]] = i;  // M[i] < fd: a first appearance is unique
     __syncthreads();
-    const int cnt = nkeys;
-    int len = 1;
-    while (len < cnt) len <<= 1;
-    for (int i = cnt + lane; i < len; i += 64) S[i] = ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= len; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < len; i += 64) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const unsigned long long a = S[i], b = S[l];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        S[i] = b;
-                        S[l] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
     const int off = tpos[p];
-    for (int i = lane; i < cnt; i += 64) {
-        const int slot = (int)(S[i] & 0xffffffffu);
-        tq[off + i] = K[slot];
-        tw[off + i] = V[slot];
+    int cnt = 0;
+    for (int c = 0; c < fd; c += 64) {
+        const int t = c + lane;
+        const int slot = t < fd ? stage[t] : -1;
+        const unsigned long long m = __ballot(slot >= 0);
+        if (slot >= 0) {
+            const int at = off + cnt + __popcll(m & ((1ull << lane) - 1));
+            tq[at] = K[slot];
+            tw[at] = V[slot];
+        }
+        cnt += __popcll(m);
     }
     if (lane == 0) cdeg[p] = cnt;
 }
 
-// Rows with more distinct neighbours than the wave tier holds (kRowCap / 2): one workgroup of
-// THREADS per row (grid-stride over the list), an LDS hash of up to CAP slots and an LDS bitonic
-// sort of up to MAXD entries; rows with more distinct neighbours are passed on to the next tier.
-// The members' edges are expanded with a load-balanced scan, MAXD members at a time: S holds
-// each member's (inclusive degree prefix, row start), every thread takes every THREADS-th edge
-// of the batch and gathers kEpl targets at a time, so the target and cluster loads of many
-// edges are in flight together. After the hash phase S holds the sort keys.
-template <int THREADS, int CAP, int MAXD>
-__global__ __launch_bounds__(THREADS) void rows_lds_big(
+// One workgroup of THREADS per row of fd <= CAP / 2 (grid-stride over the list). The members
+// are expanded MAXD at a time: S holds each member's (inclusive degree prefix, row start), every
+// thread takes every THREADS-th edge of the batch and gathers kEpl targets at a time, so the
+// target and cluster loads of many edges are in flight together. After the hash phase S holds
+// the stage (fd <= MAXD ints).
+template <int THREADS, int CAP>
+__global__ __launch_bounds__(THREADS) void rows_block(
     const int* __restrict__ list, const int* __restrict__ list_count, const int* __restrict__ xadj,
     const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
     const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ fdeg,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
-    int* __restrict__ tw, int* __restrict__ overflow_list, int* __restrict__ overflow_count,
-    int* __restrict__ err) {
-    static_assert(CAP == 2 * MAXD && MAXD % THREADS == 0, "table shape");
+    int* __restrict__ tw, int* __restrict__ err) {
+    constexpr int MAXD = CAP / 2;
     constexpr int kEpl = 4;
-    constexpr int R = MAXD / THREADS;  // members per thread in the prefix scan
-    static_assert(MAXD + THREADS * kEpl <= CAP, "rows_lds_big table too small");
+    constexpr int R = MAXD / THREADS;  // members per thread in a batch's prefix scan
+    static_assert(MAXD % THREADS == 0, "table shape");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
     __shared__ int M[CAP];
     __shared__ unsigned long long S[MAXD];
     __shared__ int wsum[THREADS / 64];
-    __shared__ int used, over, nkeys;
+    int* const stage = reinterpret_cast<int*>(S);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int count = *list_count;
     for (int li = blockIdx.x; li < count; li += gridDim.x) {
         const int p = list[li];
-        const int fd = fdeg[p];
-        int cap = 64;  // distinct <= fd
-        while (cap < 2 * fd && cap < CAP) cap <<= 1;
+        const int fd = fdeg[p];  // <= MAXD (row_bins)
+        int cap = 64;
+        while (cap < 2 * fd) cap <<= 1;
         for (int i = threadIdx.x; i < cap; i += THREADS) {
             K[i] = -1;
             V[i] = 0;
             M[i] = 0x7fffffff;
         }
-        if (threadIdx.x == 0) {
-            used = 0;
-            over = 0;
-            nkeys = 0;
-        }
         __syncthreads();
         const int mb = moff[p], me = moff[p + 1];
+        int tbase = 0;
         for (int base = mb; base < me; base += MAXD) {
             // (inclusive degree prefix, row start) of the batch's members
             const int nb = min(MAXD, me - base);
@@ -353,7 +305,6 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
             }
             __syncthreads();
             for (int t0 = 0; t0 < total; t0 += THREADS * kEpl) {
-                if (lds_flag(&over)) break;  // no barrier in this loop
                 int es[kEpl], qs[kEpl], ws[kEpl];
 #pragma unroll
                 for (int j = 0; j < kEpl; ++j) {
@@ -381,62 +332,57 @@ __global__ __launch_bounds__(THREADS) void rows_lds_big(
                 for (int j = 0; j < kEpl; ++j) {
                     const int q = qs[j];
                     if (es[j] < 0 || q == p) continue;
-                    if (lds_flag(&over)) break;
-                    // at most MAXD distinct before `over`, plus THREADS x kEpl in flight: <= CAP
-                    bool fresh;
-                    const int h = probe_claim(K, cap, q, fresh);
+                    const int h = probe_claim(K, cap, q);  // distinct <= fd <= cap / 2
                     if (h < 0) {
                         *err = kErrProbe;
                         continue;
                     }
-                    if (fresh && atomicAdd(&used, 1) >= MAXD) lds_raise(&over);
                     atomicAdd(&V[h], ws[j]);
-                    atomicMin(&M[h], es[j]);
+                    atomicMin(&M[h], tbase + t0 + (int)threadIdx.x + THREADS * j);
                 }
             }
-            __syncthreads();  // S is rewritten by the next batch; over is read below
-            if (over) break;  // uniform after the barrier
+            tbase += total;
+            __syncthreads();  // S is rewritten by the next batch (or becomes the stage)
         }
-        if (over) {
-            if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_count, 1)] = p;
-            __syncthreads();
-            continue;
-        }
-        for (int i = threadIdx.x; i < cap; i += THREADS) {
-            if (K[i] >= 0) {
-                const int idx = atomicAdd(&nkeys, 1);
-                S[idx] = ((unsigned long long)(uint32_t)M[i] << 32) | (uint32_t)i;
-            }
-        }
+        for (int t = threadIdx.x; t < fd; t += THREADS) stage[t] = -1;
         __syncthreads();
-        const int cnt = nkeys;
-        int len = 1;
-        while (len < cnt) len <<= 1;
-        for (int i = cnt + threadIdx.x; i < len; i += THREADS) S[i] = ~0ull;
+        for (int i = threadIdx.x; i < cap; i += THREADS)
+            if (K[i] >= 0) stage[M[i]] = i;  // M[i] < fd: a first appearance is unique
         __syncthreads();
-        for (int k = 2; k <= len; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = threadIdx.x; i < len; i += THREADS) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const unsigned long long a = S[i], b = S[l];
-                        if ((a > b) == ((i & k) == 0)) {
-                            S[i] = b;
-                            S[l] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        // ordered compaction: thread k owns t in [k PER, k PER + PER), then a workgroup scan
+        constexpr int PMAX = MAXD / THREADS;
+        const int per = (fd + THREADS - 1) / THREADS;  // <= PMAX
+        int sl[PMAX];
+        int mine = 0;
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const int t = (int)threadIdx.x * per + j;
+            sl[j] = j < per && t < fd ? stage[t] : -1;
+            mine += sl[j] >= 0;
         }
-        const int off = tpos[p];
-        for (int i = threadIdx.x; i < cnt; i += THREADS) {
-            const int slot = (int)(S[i] & 0xffffffffu);
-            tq[off + i] = K[slot];
-            tw[off + i] = V[slot];
+        int inc2 = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc2, o, 64);
+            if (lane >= o) inc2 += y;
+        }
+        if (lane == 63) wsum[wave] = inc2;
+        __syncthreads();
+        int at = inc2 - mine, cnt = 0;
+        for (int w = 0; w < THREADS / 64; ++w) {
+            if (w < wave) at += wsum[w];
+            cnt += wsum[w];
+        }
+        at += tpos[p];
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            if (sl[j] < 0) continue;
+            tq[at] = K[sl[j]];
+            tw[at] = V[sl[j]];
+            ++at;
         }
         if (threadIdx.x == 0) cdeg[p] = cnt;
-        __syncthreads();
+        __syncthreads();  // the tables, the stage and wsum are reused by the next row
     }
 }
 
@@ -760,15 +706,6 @@ __global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
     totals[t] = posT[ct_base[ri] + (long long)(b + 1) * nch] - posT[ct_base[ri] + (long long)b * nch];
 }
 
-__global__ void member_segments(const int* __restrict__ list, int count,
-                                const int* __restrict__ moff, int* __restrict__ beg,
-                                int* __restrict__ end) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    beg[i] = moff[list[i]];
-    end[i] = moff[list[i] + 1];
-}
-
 // The f range [f0, f1) of global chunk `chunk` (row ri of the list).
 __device__ inline void chunk_range(const int* __restrict__ list, const int* __restrict__ chunk_prefix,
                                    const int* __restrict__ moff, const int* __restrict__ mpre,
@@ -922,75 +859,81 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
     fdeg_.reserve(nc + 1);
     moff_.reserve(nc + 1);
     tpos_.reserve(nc + 1);
-    cursor_.reserve(nc);
     cdeg_.reserve(nc + 1);
     members_.reserve(n);
     fill_async(cnt_.p, nc + 1, 0, s);
     fill_async(fdeg_.p, nc + 1, 0, s);
-    fill_async(cursor_.p, nc, 0, s);
     fill_async(cdeg_.p, nc + 1, 0, s);
     fill_async(out.g.vwgt, nc, 0, s);
     if (n > 0) {
         hipLaunchKernelGGL(count_members, dim3(grid_for(n, 256)), dim3(256), 0, s, n, g.xadj,
                            g.vwgt, map, cnt_.p, fdeg_.p, out.g.vwgt);
+        // members grouped by cluster, ascending node id inside a group (a stable radix sort of
+        // the node ids by cluster)
+        iota_.reserve(n);
+        mkeys_.reserve(n);
+        hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, iota_.p);
+        int bits = 1;
+        while (bits < 31 && (1LL << bits) < (long long)nc) ++bits;
+        const unsigned* keys = reinterpret_cast<const unsigned*>(map);
+        unsigned* keys_out = reinterpret_cast<unsigned*>(mkeys_.p);
+        size_t bytes = 0;
+        KH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys_out, iota_.p, members_.p,
+                                           (unsigned)n, 0, bits, s));
+        char* t = scan_tmp_.reserve((long long)bytes + 1);
+        KH_CHECK(rocprim::radix_sort_pairs(t, bytes, keys, keys_out, iota_.p, members_.p,
+                                           (unsigned)n, 0, bits, s));
     }
     scan_i32(cnt_.p, moff_.p, nc + 1, s);
     scan_i32(fdeg_.p, tpos_.p, nc + 1, s);
-    if (n > 0) {
-        hipLaunchKernelGGL(scatter_members, dim3(grid_for(n, 256)), dim3(256), 0, s, n, map,
-                           moff_.p, cursor_.p, members_.p);
-    }
     const long long m2 = g.m2;
     tq_.reserve(m2);
     tw_.reserve(m2);
-    ovl_list_.reserve(nc);
-    ovl_count_.reserve(3);
     ovl2_list_.reserve(nc);
-    ovl3_list_.reserve(nc);
-    fill_async(ovl_count_.p, 3, 0, s);
+    ovl_count_.reserve(1);
+    fill_async(ovl_count_.p, 1, 0, s);
     err_.reserve(1);
     fill_async(err_.p, 1, 0, s);
     if (nc > 0) {
         // algorithmic bytes of the row pass: 12 B per fine edge (target, weight, cluster of
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
         const double bytes = 12.0 * (double)m2 + 12.0 * n;
-        rsmall_.reserve(nc);
-        rmid_.reserve(nc);
-        rcount_.reserve(2);
-        fill_async(rcount_.p, 2, 0, s);
+        for (auto& l : rlist_) l.reserve(nc);
+        rcount_.reserve(4);
+        fill_async(rcount_.p, 4, 0, s);
         hipLaunchKernelGGL(row_bins, dim3(grid_for(nc, 256)), dim3(256), 0, s, nc, fdeg_.p,
-                           rsmall_.p, rmid_.p, rcount_.p, ovl2_list_.p, ovl_count_.p + 1);
+                           rlist_[0].p, rlist_[1].p, rlist_[2].p, rlist_[3].p, rcount_.p,
+                           ovl2_list_.p, ovl_count_.p);
         int rc[2] = {0, 0};
         KH_CHECK(hipMemcpyAsync(rc, rcount_.p, sizeof(rc), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
-        ProfScope ps(prof, "contract_rows_lds", bytes, m2, s);
-        if (rc[0] > 0)
-            hipLaunchKernelGGL((rows_lds<2 * kRowSmall, false>), dim3(rc[0]), dim3(64), 0, s,
-                               rsmall_.p, rc[0], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
-                               members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
-                               ovl_count_.p, err_.p);
-        if (rc[1] > 0)
-            hipLaunchKernelGGL((rows_lds<kRowCap, true>), dim3(rc[1]), dim3(64), 0, s, rmid_.p,
-                               rc[1], g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p,
-                               fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl_list_.p,
-                               ovl_count_.p, err_.p);
-    }
-    if (nc > 0) {
-        // overflow of rows_lds -> mid tier (<= kMidMaxDistinct) -> big tier (<= kBigMaxDistinct)
-        // -> global tier (ovl2 list, which also holds the rows of more than kDirectGlobal edges)
-        ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);
-        hipLaunchKernelGGL((rows_lds_big<kMidThreads, 2 * kMidMaxDistinct, kMidMaxDistinct>),
-                           dim3(256 * 5), dim3(kMidThreads), 0, s, ovl_list_.p, ovl_count_.p,
-                           g.xadj, g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p,
-                           tpos_.p, cdeg_.p, tq_.p, tw_.p, ovl3_list_.p, ovl_count_.p + 2,
-                           err_.p);
-        hipLaunchKernelGGL((rows_lds_big<kBigThreads, kBigCap, kBigMaxDistinct>), dim3(256),
-                           dim3(kBigThreads), 0, s, ovl3_list_.p, ovl_count_.p + 2, g.xadj,
-                           g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p, tpos_.p,
-                           cdeg_.p, tq_.p, tw_.p, ovl2_list_.p, ovl_count_.p + 1, err_.p);
+        {
+            ProfScope ps(prof, "contract_rows_lds", bytes, m2, s);
+            if (rc[0] > 0)
+                hipLaunchKernelGGL((rows_wave<kWaveTierCap[0]>), dim3(rc[0]), dim3(64), 0, s,
+                                   rlist_[0].p, rc[0], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
+                                   members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, err_.p);
+            if (rc[1] > 0)
+                hipLaunchKernelGGL((rows_wave<kWaveTierCap[1]>), dim3(rc[1]), dim3(64), 0, s,
+                                   rlist_[1].p, rc[1], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
+                                   members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, err_.p);
+        }
+        {
+            // grid-stride over the device-side counts: five 256-thread workgroups per CU, one
+            // 1024-thread workgroup per CU
+            ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);
+            hipLaunchKernelGGL((rows_block<kMidThreads, kMidCap>), dim3(256 * 5),
+                               dim3(kMidThreads), 0, s, rlist_[2].p, rcount_.p + 2, g.xadj,
+                               g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p, tpos_.p,
+                               cdeg_.p, tq_.p, tw_.p, err_.p);
+            hipLaunchKernelGGL((rows_block<kBigThreads, kBigCap>), dim3(256), dim3(kBigThreads),
+                               0, s, rlist_[3].p, rcount_.p + 3, g.xadj, g.adjncy, g.adjwgt, map,
+                               moff_.p, members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p,
+                               err_.p);
+        }
     }
     int novl = 0;
-    KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    KH_CHECK(hipMemcpyAsync(&novl, ovl_count_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     KH_CHECK(hipStreamSynchronize(s));
     if (novl > 0) {
         ProfScope ps(prof, "contract_rows_global", 0.0, novl, s);
@@ -1041,38 +984,17 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         gnbig_.reserve(1);
         fill_async(gcnt_.p + ct_total, 1, 0, s);  // sentinel
         fill_async(gnbig_.p, 1, 0, s);
-        // the members of the global rows in ascending node id (the other rows keep theirs), so
-        // that a row's flattened edge index f grows with the CSR index e
-        members_s_.reserve(std::max(n, 1));
-        mseg_.reserve(2LL * novl);
-        if (n > 0)
-            KH_CHECK(hipMemcpyAsync(members_s_.p, members_.p, sizeof(int) * n,
-                                    hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(member_segments, dim3(grid_for(novl, 256)), dim3(256), 0, s,
-                           ovl2_list_.p, novl, moff_.p, mseg_.p, mseg_.p + novl);
-        {
-            int bits = 1;
-            while (bits < 31 && (1LL << bits) < (long long)n) ++bits;
-            size_t bytes = 0;
-            KH_CHECK(rocprim::segmented_radix_sort_keys(nullptr, bytes, members_.p,
-                                                        members_s_.p, (unsigned)n, (unsigned)novl,
-                                                        mseg_.p, mseg_.p + novl, 0, bits, s));
-            char* t = scan_tmp_.reserve((long long)bytes + 1);
-            KH_CHECK(rocprim::segmented_radix_sort_keys(t, bytes, members_.p, members_s_.p,
-                                                        (unsigned)n, (unsigned)novl, mseg_.p,
-                                                        mseg_.p + novl, 0, bits, s));
-        }
         mdeg_.reserve((long long)n + 1);
         mpre_.reserve((long long)n + 1);
         hipLaunchKernelGGL(member_degrees, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
-                           members_s_.p, g.xadj, mdeg_.p);
+                           members_.p, g.xadj, mdeg_.p);
         fill_async(mdeg_.p + n, 1, 0, s);
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
         stage_q_.reserve(std::max<long long>(m2, 1));
         stage_w_.reserve(std::max<long long>(m2, 1));
         hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
                            gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
-                           members_s_.p, mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p,
+                           members_.p, mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p,
                            gpart_.p, stage_q_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
         hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,

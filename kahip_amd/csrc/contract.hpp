@@ -36,16 +36,16 @@ private:
     void scan_i32(const int* in, int* out, int n, hipStream_t s);
     DevBuf<char> scan_tmp_;
     DevBuf<int> leader_, flag_, newid_;
-    DevBuf<int> cnt_, fdeg_, moff_, tpos_, cursor_, cdeg_, members_, tq_, tw_;
-    DevBuf<int> ovl_list_, ovl2_list_, ovl3_list_, ovl_count_;
-    DevBuf<int> rsmall_, rmid_, rcount_;
+    DevBuf<int> cnt_, fdeg_, moff_, tpos_, cdeg_, members_, tq_, tw_;
+    DevBuf<int> iota_, mkeys_;               // radix sort of the node ids by cluster
+    DevBuf<int> rlist_[4], rcount_;          // rows of the four LDS tiers (by fd)
+    DevBuf<int> ovl2_list_, ovl_count_;      // rows of the global tier
     DevBuf<long long> hcap_, hoff_;
     DevBuf<int> gchunk_prefix_, mdeg_, mpre_, ofd_;
     DevBuf<int> hk_, hv_, hm_;
     DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, gbig_, gnbig_;
     DevBuf<long long> goffm_base_, gct_base_;
     DevBuf<int4> gpart_, gbuck_;
-    DevBuf<int> members_s_, mseg_;           // global rows' members sorted; segment bounds
     DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
     DevBuf<int> ecnt_, epos_;                // staged entries per chunk and their scan
     DevBuf<int> err_;  // device error flag (kErrProbe) of the last contract()
