@@ -13,6 +13,7 @@
 //   count       coarse weights, member counts, summed member degrees fd (row capacity bound);
 //               the members of every coarse node in ascending id (a stable radix sort by
 //               cluster), so that the row's edge t = 0, 1, ... grows with the fine edge index.
+//   rows_thread one thread per row of fd <= 8 (rows of fd = 0 need no work);
 //   rows_wave   one wave per row of fd <= 128 / <= 512: LDS hash (q -> w, min t), a stage
 //               indexed by t that marks each entry at its first appearance, one ordered pass;
 //   rows_block  the same with one workgroup per row of fd <= 1024 (256 threads) / <= 4096 (1024);
@@ -42,6 +43,7 @@ constexpr int kMidThreads = 256;     // one-workgroup tiers: rows of fd <= 1024 
 constexpr int kMidCap = 2 * 1024;    // 8 KB of member staging, 5 per CU) ...
 constexpr int kBigThreads = 1024;    // ... and of fd <= 4096 (96 KB + 32 KB)
 constexpr int kBigCap = 2 * 4096;
+constexpr int kThreadTierFd = 8;                    // largest fd of the thread tier
 constexpr int kTierFd[4] = {128, 512, 1024, 4096};  // largest fd of each LDS tier
 static_assert(2 * kTierFd[0] == kWaveTierCap[0] && 2 * kTierFd[1] == kWaveTierCap[1] &&
                   2 * kTierFd[2] == kMidCap && 2 * kTierFd[3] == kBigCap,
@@ -87,34 +89,80 @@ __global__ void iota_kernel(int n, int* __restrict__ out) {
 }
 
 // Rows by summed member degree fd (an upper bound of the row's distinct neighbours), so that no
-// tier can overflow: fd <= 128 and fd <= 512 go to the one-wave tiers, fd <= 1024 and fd <= 4096
-// to the one-workgroup tiers, larger rows to the chunked global tier. Workgroup-aggregated list
-// appends.
+// tier can overflow: fd = 0 needs no work (cdeg stays 0); fd <= 8 goes to the thread tier,
+// fd <= 128 and fd <= 512 to the one-wave tiers, fd <= 1024 and fd <= 4096 to the
+// one-workgroup tiers, larger rows to the chunked global tier. Workgroup-aggregated list appends.
 __global__ __launch_bounds__(256) void row_bins(int nc, const int* __restrict__ fdeg,
                                                 int* __restrict__ l0, int* __restrict__ l1,
                                                 int* __restrict__ l2, int* __restrict__ l3,
-                                                int* __restrict__ counts,
+                                                int* __restrict__ l4, int* __restrict__ counts,
                                                 int* __restrict__ global_list,
                                                 int* __restrict__ global_count) {
-    __shared__ int lc[5], bc[5];
-    if (threadIdx.x < 5) lc[threadIdx.x] = 0;
+    __shared__ int lc[6], bc[6];
+    if (threadIdx.x < 6) lc[threadIdx.x] = 0;
     __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1, rank = 0;
     if (p < nc) {
         const int fd = fdeg[p];
-        cls = fd <= kTierFd[0] ? 0 : fd <= kTierFd[1] ? 1 : fd <= kTierFd[2] ? 2 : fd <= kTierFd[3] ? 3 : 4;
-        rank = atomicAdd(&lc[cls], 1);
+        if (fd > 0) {
+            cls = fd <= kThreadTierFd ? 0 : fd <= kTierFd[0] ? 1 : fd <= kTierFd[1] ? 2
+                : fd <= kTierFd[2] ? 3 : fd <= kTierFd[3] ? 4 : 5;
+            rank = atomicAdd(&lc[cls], 1);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < 6) {
         const int c = lc[threadIdx.x];
-        bc[threadIdx.x] = c ? atomicAdd(threadIdx.x < 4 ? &counts[threadIdx.x] : global_count, c) : 0;
+        bc[threadIdx.x] = c ? atomicAdd(threadIdx.x < 5 ? &counts[threadIdx.x] : global_count, c) : 0;
     }
     __syncthreads();
     if (cls < 0) return;
-    int* const lists[5] = {l0, l1, l2, l3, global_list};
+    int* const lists[6] = {l0, l1, l2, l3, l4, global_list};
     lists[cls][bc[cls] + rank] = p;
+}
+
+// One thread per row of fd <= kThreadTierFd: the members (ascending id) and their edges are
+// scanned in order, so a neighbour cluster is appended at its first appearance and later
+// appearances add their weight (a linear search of at most kThreadTierFd entries, kept in LDS
+// columns per thread).
+__global__ __launch_bounds__(256) void rows_thread(
+    const int* __restrict__ rows, int nrows, const int* __restrict__ xadj,
+    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
+    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ tpos,
+    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw) {
+    __shared__ int Q[kThreadTierFd][256];
+    __shared__ int W[kThreadTierFd][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nrows) return;  // no barrier below
+    const int p = rows[i];
+    const int t = threadIdx.x;
+    int cnt = 0;  // distinct <= fd <= kThreadTierFd
+    const int me = moff[p + 1];
+    for (int m = moff[p]; m < me; ++m) {
+        const int u = members[m];
+        const int e1 = xadj[u + 1];
+        for (int e = xadj[u]; e < e1; ++e) {
+            const int q = map[adjncy[e]];
+            if (q == p) continue;
+            const int w = adjwgt[e];
+            int j = 0;
+            while (j < cnt && Q[j][t] != q) ++j;
+            if (j == cnt) {
+                Q[j][t] = q;
+                W[j][t] = w;
+                ++cnt;
+            } else {
+                W[j][t] += w;
+            }
+        }
+    }
+    const int off = tpos[p];
+    for (int j = 0; j < cnt; ++j) {
+        tq[off + j] = Q[j][t];
+        tw[off + j] = W[j][t];
+    }
+    cdeg[p] = cnt;
 }
 
 // The two row kernels below aggregate a coarse row p in LDS: its members (ascending node id)
@@ -430,7 +478,7 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
     const int* __restrict__ mpre, const long long* __restrict__ offm_base, int* __restrict__ offm,
     const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials,
-    int* __restrict__ stage_q, int* __restrict__ err) {
+    unsigned* __restrict__ smark, int* __restrict__ err) {
     __shared__ int K[kGCap];
     __shared__ int V[kGCap];
     __shared__ int M[kGCap];
@@ -480,7 +528,11 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     __syncthreads();
     const int m0 = m0s, nm = nms;
     const bool staged = nm <= kGChunk;  // uniform
-    for (long long f = f0 + threadIdx.x; f < f1; f += kGThreads) stage_q[f] = -1;
+    // clear the stage marks of [f0, f1) (words shared with a neighbouring range get the same 0;
+    // marks are set only by the merge kernels, after this one)
+    if (f1 > f0)
+        for (long long w = (f0 >> 5) + threadIdx.x; w <= ((f1 - 1) >> 5); w += kGThreads)
+            smark[w] = 0u;
     if (staged)
         for (int i = threadIdx.x; i < nm; i += kGThreads) {
             sbeg[i] = mpre[m0 + i];
@@ -601,8 +653,9 @@ __global__ __launch_bounds__(256) void gscatter(
 __global__ __launch_bounds__(512) void gmerge(
     const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix, int count,
     int n_tasks, const long long* __restrict__ ct_base, const int* __restrict__ posT,
-    const int4* __restrict__ bucketed, int* __restrict__ stage_q, int* __restrict__ stage_w,
-    int* __restrict__ big_tasks, int* __restrict__ n_big, int* __restrict__ err) {
+    const int4* __restrict__ bucketed, unsigned* __restrict__ smark, int* __restrict__ stage_q,
+    int* __restrict__ stage_w, int* __restrict__ big_tasks, int* __restrict__ n_big,
+    int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
@@ -643,8 +696,10 @@ __global__ __launch_bounds__(512) void gmerge(
     for (int i = threadIdx.x; i < cap; i += 512) {
         const int q = K[i];
         if (q < 0) continue;
-        stage_q[M[i]] = q;  // one merged entry per (row, q): its min f is unique
-        stage_w[M[i]] = V[i];
+        const int f = M[i];  // one merged entry per (row, q): its min f is unique
+        atomicOr(&smark[f >> 5], 1u << (f & 31));
+        stage_q[f] = q;
+        stage_w[f] = V[i];
     }
 }
 
@@ -655,8 +710,8 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     const int* __restrict__ task_prefix, int count, const long long* __restrict__ ct_base,
     const int* __restrict__ posT, const int4* __restrict__ bucketed,
     const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
-    int* __restrict__ TV, int* __restrict__ TM, int* __restrict__ stage_q,
-    int* __restrict__ stage_w, int* __restrict__ err) {
+    int* __restrict__ TV, int* __restrict__ TM, unsigned* __restrict__ smark,
+    int* __restrict__ stage_q, int* __restrict__ stage_w, int* __restrict__ err) {
     __shared__ int row;
     const int t = blockIdx.x;
     if (t >= n_big) return;
@@ -687,8 +742,10 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     for (long long i = threadIdx.x; i < cap; i += 1024) {
         const int q = K[i];
         if (q < 0) continue;
-        stage_q[Mn[i]] = q;
-        stage_w[Mn[i]] = V[i];
+        const int f = Mn[i];
+        atomicOr(&smark[f >> 5], 1u << (f & 31));
+        stage_q[f] = q;
+        stage_w[f] = V[i];
     }
 }
 
@@ -719,10 +776,19 @@ __device__ inline void chunk_range(const int* __restrict__ list, const int* __re
 constexpr int kEmitThreads = 256;  // kGChunk / kEmitThreads = 16 staged slots per thread
 static_assert(kGChunk % kEmitThreads == 0, "every emission thread takes whole slots");
 
-// Staged entries of each chunk (one workgroup per chunk).
+// Marked bits of [f0, f1) in word w of the stage marks.
+__device__ inline unsigned marks_in(const unsigned* __restrict__ smark, long long w, long long f0,
+                                    long long f1) {
+    const long long b0 = w << 5;
+    const int lo = (int)(max(f0, b0) - b0), hi = (int)(min(f1, b0 + 32) - b0);  // 0 <= lo < hi <= 32
+    const unsigned m = (hi == 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+    return smark[w] & m;
+}
+
+// Staged entries of each chunk (one workgroup per chunk): marked bits of its f range.
 __global__ __launch_bounds__(kEmitThreads) void gemit_count(
     const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ moff, const int* __restrict__ mpre, const int* __restrict__ stage_q,
+    const int* __restrict__ moff, const int* __restrict__ mpre, const unsigned* __restrict__ smark,
     int* __restrict__ ecnt) {
     __shared__ int row, wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
@@ -731,7 +797,9 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_count(
     long long f0, f1;
     chunk_range(list, chunk_prefix, moff, mpre, row, chunk, f0, f1);
     int c = 0;
-    for (long long f = f0 + threadIdx.x; f < f1; f += kEmitThreads) c += stage_q[f] >= 0;
+    if (f1 > f0)
+        for (long long w = (f0 >> 5) + threadIdx.x; w <= ((f1 - 1) >> 5); w += kEmitThreads)
+            c += __popc(marks_in(smark, w, f0, f1));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
@@ -748,10 +816,12 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_count(
 // takes the chunk's slots [16 t, 16 t + 16), so ranks follow f.
 __global__ __launch_bounds__(kEmitThreads) void gemit_write(
     const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ moff, const int* __restrict__ mpre, const int* __restrict__ stage_q,
-    const int* __restrict__ stage_w, const int* __restrict__ epos, const int* __restrict__ tpos,
-    int* __restrict__ cdeg, int* __restrict__ tq, int* __restrict__ tw) {
+    const int* __restrict__ moff, const int* __restrict__ mpre, const unsigned* __restrict__ smark,
+    const int* __restrict__ stage_q, const int* __restrict__ stage_w, const int* __restrict__ epos,
+    const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
+    int* __restrict__ tw) {
     constexpr int PER = kGChunk / kEmitThreads;
+    static_assert(PER <= 32, "a thread's slots span at most two mark words");
     __shared__ int row, wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
     if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
@@ -761,13 +831,16 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_write(
     long long f0, f1;
     chunk_range(list, chunk_prefix, moff, mpre, ri, chunk, f0, f1);
     const long long a0 = f0 + (long long)threadIdx.x * PER;
-    int qs[PER];
-    int mine = 0;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        qs[j] = a0 + j < f1 ? stage_q[a0 + j] : -1;
-        mine += qs[j] >= 0;
+    // marks of the thread's slots [a0, min(a0 + PER, f1)), bit j = slot a0 + j
+    unsigned bits = 0u;
+    if (a0 < f1) {
+        const long long a1 = min(f1, a0 + PER);
+        const long long w0 = a0 >> 5, w1 = (a1 - 1) >> 5;
+        const int sh = (int)(a0 & 31);
+        bits = marks_in(smark, w0, a0, a1) >> sh;
+        if (w1 != w0) bits |= marks_in(smark, w1, a0, a1) << (32 - sh);
     }
+    const int mine = __popc(bits);
     int incl = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -780,10 +853,10 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_write(
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wsum[w];
     const int first = chunk_prefix[ri];
     int at = tpos[p] + (epos[chunk] - epos[first]) + before;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        if (qs[j] < 0) continue;
-        tq[at] = qs[j];
+    while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        tq[at] = stage_q[a0 + j];
         tw[at] = stage_w[a0 + j];
         ++at;
     }
@@ -899,23 +972,27 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         // target), 12 B per fine node (offsets, member id), 8 B per coarse entry written
         const double bytes = 12.0 * (double)m2 + 12.0 * n;
         for (auto& l : rlist_) l.reserve(nc);
-        rcount_.reserve(4);
-        fill_async(rcount_.p, 4, 0, s);
+        rcount_.reserve(5);
+        fill_async(rcount_.p, 5, 0, s);
         hipLaunchKernelGGL(row_bins, dim3(grid_for(nc, 256)), dim3(256), 0, s, nc, fdeg_.p,
-                           rlist_[0].p, rlist_[1].p, rlist_[2].p, rlist_[3].p, rcount_.p,
-                           ovl2_list_.p, ovl_count_.p);
-        int rc[2] = {0, 0};
+                           rlist_[0].p, rlist_[1].p, rlist_[2].p, rlist_[3].p, rlist_[4].p,
+                           rcount_.p, ovl2_list_.p, ovl_count_.p);
+        int rc[3] = {0, 0, 0};
         KH_CHECK(hipMemcpyAsync(rc, rcount_.p, sizeof(rc), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
         {
             ProfScope ps(prof, "contract_rows_lds", bytes, m2, s);
             if (rc[0] > 0)
-                hipLaunchKernelGGL((rows_wave<kWaveTierCap[0]>), dim3(rc[0]), dim3(64), 0, s,
+                hipLaunchKernelGGL(rows_thread, dim3(grid_for(rc[0], 256)), dim3(256), 0, s,
                                    rlist_[0].p, rc[0], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
-                                   members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, err_.p);
+                                   members_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
             if (rc[1] > 0)
-                hipLaunchKernelGGL((rows_wave<kWaveTierCap[1]>), dim3(rc[1]), dim3(64), 0, s,
+                hipLaunchKernelGGL((rows_wave<kWaveTierCap[0]>), dim3(rc[1]), dim3(64), 0, s,
                                    rlist_[1].p, rc[1], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
+                                   members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, err_.p);
+            if (rc[2] > 0)
+                hipLaunchKernelGGL((rows_wave<kWaveTierCap[1]>), dim3(rc[2]), dim3(64), 0, s,
+                                   rlist_[2].p, rc[2], g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
                                    members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p, err_.p);
         }
         {
@@ -923,11 +1000,11 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             // 1024-thread workgroup per CU
             ProfScope ps(prof, "contract_rows_lds_big", 0.0, 0, s);
             hipLaunchKernelGGL((rows_block<kMidThreads, kMidCap>), dim3(256 * 5),
-                               dim3(kMidThreads), 0, s, rlist_[2].p, rcount_.p + 2, g.xadj,
+                               dim3(kMidThreads), 0, s, rlist_[3].p, rcount_.p + 3, g.xadj,
                                g.adjncy, g.adjwgt, map, moff_.p, members_.p, fdeg_.p, tpos_.p,
                                cdeg_.p, tq_.p, tw_.p, err_.p);
             hipLaunchKernelGGL((rows_block<kBigThreads, kBigCap>), dim3(256), dim3(kBigThreads),
-                               0, s, rlist_[3].p, rcount_.p + 3, g.xadj, g.adjncy, g.adjwgt, map,
+                               0, s, rlist_[4].p, rcount_.p + 4, g.xadj, g.adjncy, g.adjwgt, map,
                                moff_.p, members_.p, fdeg_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p,
                                err_.p);
         }
@@ -992,16 +1069,17 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
         stage_q_.reserve(std::max<long long>(m2, 1));
         stage_w_.reserve(std::max<long long>(m2, 1));
+        smark_.reserve(m2 / 32 + 2);
         hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
                            gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
                            members_.p, mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p,
-                           gpart_.p, stage_q_.p, err_.p);
+                           gpart_.p, smark_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
         hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,
                            goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
         hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(512), 0, s, gchunk_prefix_.p,
                            gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p,
-                           stage_q_.p, stage_w_.p, gbig_.p, gnbig_.p, err_.p);
+                           smark_.p, stage_q_.p, stage_w_.p, gbig_.p, gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -1039,8 +1117,8 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             fill_async(hm_.p, ttotal, 0x7fffffff, s);
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
                                gchunk_prefix_.p, gtask_prefix_.p, novl, gct_base_.p, gpos_.p,
-                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, stage_q_.p,
-                               stage_w_.p, err_.p);
+                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, smark_.p,
+                               stage_q_.p, stage_w_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
         }
         // emission in f order: counts per chunk, a scan over the chunks, ordered writes
@@ -1048,11 +1126,11 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         epos_.reserve((long long)nchunks + 1);
         fill_async(ecnt_.p + nchunks, 1, 0, s);
         hipLaunchKernelGGL(gemit_count, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
-                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, stage_q_.p, ecnt_.p);
+                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, smark_.p, ecnt_.p);
         scan_i32(ecnt_.p, epos_.p, nchunks + 1, s);
         hipLaunchKernelGGL(gemit_write, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
-                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, stage_q_.p, stage_w_.p,
-                           epos_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
+                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, smark_.p, stage_q_.p,
+                           stage_w_.p, epos_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
     }
     scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
     int mc = 0, err = 0;

@@ -38,7 +38,7 @@ private:
     DevBuf<int> leader_, flag_, newid_;
     DevBuf<int> cnt_, fdeg_, moff_, tpos_, cdeg_, members_, tq_, tw_;
     DevBuf<int> iota_, mkeys_;               // radix sort of the node ids by cluster
-    DevBuf<int> rlist_[4], rcount_;          // rows of the four LDS tiers (by fd)
+    DevBuf<int> rlist_[5], rcount_;          // rows of the thread tier and the four LDS tiers
     DevBuf<int> ovl2_list_, ovl_count_;      // rows of the global tier
     DevBuf<long long> hcap_, hoff_;
     DevBuf<int> gchunk_prefix_, mdeg_, mpre_, ofd_;
@@ -47,6 +47,7 @@ private:
     DevBuf<long long> goffm_base_, gct_base_;
     DevBuf<int4> gpart_, gbuck_;
     DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
+    DevBuf<unsigned> smark_;                 // one bit per f: an entry is staged there
     DevBuf<int> ecnt_, epos_;                // staged entries per chunk and their scan
     DevBuf<int> err_;  // device error flag (kErrProbe) of the last contract()
 };
