@@ -50,7 +50,8 @@ static_assert(2 * kTierFd[0] == kWaveTierCap[0] && 2 * kTierFd[1] == kWaveTierCa
               "every LDS tier holds rows of up to half its table in member edges");
 constexpr int kGChunk = 4096;      // member edges per workgroup in the global tier
 constexpr int kGCap = 8192;        // LDS hash slots of a global-tier chunk
-constexpr int kGThreads = 512;
+constexpr int kGThreads = 1024;  // 4 member edges per thread
+constexpr int kGMergeThreads = 1024;
 static_assert(2 * kGChunk <= kGCap, "a global-tier chunk fills at most half its table");
 
 inline int grid_for(long long work, int block) { return (int)((work + block - 1) / block); }
@@ -463,20 +464,44 @@ __device__ inline int gbucket(int q, int P) {
     return (int)(mix32((uint32_t)q ^ 0x27d4eb2fu) % (uint32_t)P);
 }
 
-__device__ inline int row_of_chunk(const int* __restrict__ chunk_prefix, int count, int chunk) {
-    int lo = 0, hi = count;  // largest i with chunk_prefix[i] <= chunk
+// Per-chunk descriptors of the global tier: da = {row index ri, chunk of the row, first member
+// overlapping the chunk m0, members overlapping it nm}, db = {coarse node p, chunks of the row,
+// f0, f1} (the chunk's edges [f0, f1) in the flattened member-edge space). One thread per chunk,
+// so the binary searches run side by side instead of serially at the start of every workgroup.
+__global__ void gchunk_desc(const int* __restrict__ crow, int nchunks, const int* __restrict__ list,
+                            const int* __restrict__ chunk_prefix, const int* __restrict__ moff,
+                            const int* __restrict__ mpre, int4* __restrict__ da,
+                            int4* __restrict__ db) {
+    const int chunk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (chunk >= nchunks) return;
+    const int ri = crow[chunk];
+    const int p = list[ri];
+    const int c0 = chunk_prefix[ri];
+    const int nch = chunk_prefix[ri + 1] - c0;
+    const int mb = moff[p], me = moff[p + 1];
+    const long long f_row = mpre[mb];
+    const long long f_end = mpre[me];
+    const long long f0 = f_row + (long long)(chunk - c0) * kGChunk;
+    const long long f1 = min(f_end, f0 + kGChunk);
+    int lo = mb, hi = me;  // largest member index with mpre <= f0
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (chunk_prefix[mid] <= chunk) lo = mid; else hi = mid;
+        if ((long long)mpre[mid] <= f0) lo = mid; else hi = mid;
     }
-    return lo;
+    int lo2 = lo, hi2 = me;  // largest member index with mpre < f1
+    while (hi2 - lo2 > 1) {
+        const int mid = (lo2 + hi2) >> 1;
+        if ((long long)mpre[mid] < f1) lo2 = mid; else hi2 = mid;
+    }
+    da[chunk] = make_int4(ri, chunk - c0, lo, lo2 - lo + 1);
+    db[chunk] = make_int4(p, nch, (int)f0, (int)f1);  // f < m2 < 2^31
 }
 
 __global__ __launch_bounds__(kGThreads) void gchunk(
-    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ xadj, const int* __restrict__ adjncy, const int* __restrict__ adjwgt,
-    const int* __restrict__ map, const int* __restrict__ moff, const int* __restrict__ members,
-    const int* __restrict__ mpre, const long long* __restrict__ offm_base, int* __restrict__ offm,
+    const int4* __restrict__ da, const int4* __restrict__ db, const int* __restrict__ xadj,
+    const int* __restrict__ adjncy, const int* __restrict__ adjwgt, const int* __restrict__ map,
+    const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ mpre,
+    const long long* __restrict__ offm_base, int* __restrict__ offm,
     const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials,
     unsigned* __restrict__ smark, int* __restrict__ err) {
     __shared__ int K[kGCap];
@@ -486,47 +511,25 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     __shared__ int wsum[kGThreads / 64];
     __shared__ int sbeg[kGChunk];  // edge-space start (mpre) of the chunk's members
     __shared__ int sst[kGChunk];   // their CSR row start
-    __shared__ int row, m0s, nms;
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
+    const int4 A = da[chunk], B = db[chunk];
     for (int i = threadIdx.x; i < kGCap; i += kGThreads) {
         K[i] = -1;
         V[i] = 0;
         M[i] = 0x7fffffff;
     }
-    __syncthreads();
-    const int ri = row;
-    const int p = list[ri];
-    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
-    const int c_idx = chunk - chunk_prefix[ri];
+    const int ri = A.x, c_idx = A.y;
+    const int p = B.x, nch = B.y;
     const int P = min(nch, kGMaxBuckets);
     const int mb = moff[p], me = moff[p + 1];
-    const long long f_row = mpre[mb];
-    const long long f_end = mpre[me];
-    const long long f0 = f_row + (long long)c_idx * kGChunk;
-    const long long f1 = min(f_end, f0 + kGChunk);
-    // The members overlapping [f0, f1) are staged in LDS (one global binary search per chunk);
-    // thread t then takes the chunk's edges t, t + kGThreads, ... and finds each one's member by
-    // a binary search in LDS, so neighbouring threads load neighbouring edges. (Members of degree
-    // 0 share their start with the next member; more than kGChunk members in a chunk can only
-    // come from those, and such a chunk searches the member prefix in global memory instead.)
+    const long long f0 = B.z, f1 = B.w;
+    // The members overlapping [f0, f1) are staged in LDS; thread t then takes the chunk's edges
+    // t, t + kGThreads, ... and finds each one's member by a binary search in LDS, so
+    // neighbouring threads load neighbouring edges. (Members of degree 0 share their start with
+    // the next member; more than kGChunk members in a chunk can only come from those, and such a
+    // chunk searches the member prefix in global memory instead.)
     constexpr int PER = kGChunk / kGThreads;
-    if (threadIdx.x == 0) {
-        int lo = mb, hi = me;  // largest member index with mpre <= f0
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if ((long long)mpre[mid] <= f0) lo = mid; else hi = mid;
-        }
-        int lo2 = lo, hi2 = me;  // largest member index with mpre < f1
-        while (hi2 - lo2 > 1) {
-            const int mid = (lo2 + hi2) >> 1;
-            if ((long long)mpre[mid] < f1) lo2 = mid; else hi2 = mid;
-        }
-        m0s = lo;
-        nms = lo2 - lo + 1;
-    }
-    __syncthreads();
-    const int m0 = m0s, nm = nms;
+    const int m0 = A.z, nm = A.w;
     const bool staged = nm <= kGChunk;  // uniform
     // clear the stage marks of [f0, f1) (words shared with a neighbouring range get the same 0;
     // marks are set only by the merge kernels, after this one)
@@ -627,17 +630,13 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
 }
 
 __global__ __launch_bounds__(256) void gscatter(
-    const int* __restrict__ chunk_prefix, int count, const long long* __restrict__ offm_base,
-    const int* __restrict__ offm, const long long* __restrict__ ct_base,
-    const int* __restrict__ posT, const int4* __restrict__ partials,
-    int4* __restrict__ bucketed) {
-    __shared__ int row;
+    const int4* __restrict__ da, const int4* __restrict__ db,
+    const long long* __restrict__ offm_base, const int* __restrict__ offm,
+    const long long* __restrict__ ct_base, const int* __restrict__ posT,
+    const int4* __restrict__ partials, int4* __restrict__ bucketed) {
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
-    __syncthreads();
-    const int ri = row;
-    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
-    const int c_idx = chunk - chunk_prefix[ri];
+    const int ri = da[chunk].x, c_idx = da[chunk].y;
+    const int nch = db[chunk].y;
     const int P = min(nch, kGMaxBuckets);
     const int* offs = offm + offm_base[ri] + (long long)c_idx * (P + 1);
     const int* pos = posT + ct_base[ri] + c_idx;
@@ -650,21 +649,18 @@ __global__ __launch_bounds__(256) void gscatter(
     }
 }
 
-__global__ __launch_bounds__(512) void gmerge(
-    const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix, int count,
-    int n_tasks, const long long* __restrict__ ct_base, const int* __restrict__ posT,
-    const int4* __restrict__ bucketed, unsigned* __restrict__ smark, int* __restrict__ stage_q,
+__global__ __launch_bounds__(kGMergeThreads) void gmerge(
+    const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix,
+    const int* __restrict__ trow, int n_tasks, const long long* __restrict__ ct_base,
+    const int* __restrict__ posT, const int4* __restrict__ bucketed, unsigned* __restrict__ smark, int* __restrict__ stage_q,
     int* __restrict__ stage_w, int* __restrict__ big_tasks, int* __restrict__ n_big,
     int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
-    __shared__ int row;
     const int task = blockIdx.x;
     if (task >= n_tasks) return;  // uniform
-    if (threadIdx.x == 0) row = row_of_chunk(task_prefix, count, task);
-    __syncthreads();
-    const int ri = row;
+    const int ri = trow[task];
     const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
     const int b = task - task_prefix[ri];
     const int r0 = posT[ct_base[ri] + (long long)b * nch];
@@ -676,13 +672,13 @@ __global__ __launch_bounds__(512) void gmerge(
     }
     int cap = 64;
     while (cap < 2 * total) cap <<= 1;
-    for (int i = threadIdx.x; i < cap; i += 512) {
+    for (int i = threadIdx.x; i < cap; i += kGMergeThreads) {
         K[i] = -1;
         V[i] = 0;
         M[i] = 0x7fffffff;
     }
     __syncthreads();
-    for (int i = r0 + threadIdx.x; i < r1; i += 512) {
+    for (int i = r0 + threadIdx.x; i < r1; i += kGMergeThreads) {
         const int4 e = bucketed[i];
         const int h = probe_claim(K, cap, e.x);  // cap >= 2 x the range's entries
         if (h < 0) {
@@ -693,7 +689,7 @@ __global__ __launch_bounds__(512) void gmerge(
         atomicMin(&M[h], e.z);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < cap; i += 512) {
+    for (int i = threadIdx.x; i < cap; i += kGMergeThreads) {
         const int q = K[i];
         if (q < 0) continue;
         const int f = M[i];  // one merged entry per (row, q): its min f is unique
@@ -707,18 +703,15 @@ __global__ __launch_bounds__(512) void gmerge(
 // tcap[t] slots at toff[t] (cleared beforehand), then the same append as gmerge.
 __global__ __launch_bounds__(1024) void gmerge_global(
     const int* __restrict__ big_tasks, int n_big, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ task_prefix, int count, const long long* __restrict__ ct_base,
-    const int* __restrict__ posT, const int4* __restrict__ bucketed,
-    const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
+    const int* __restrict__ task_prefix, const int* __restrict__ trow,
+    const long long* __restrict__ ct_base, const int* __restrict__ posT,
+    const int4* __restrict__ bucketed, const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
     int* __restrict__ TV, int* __restrict__ TM, unsigned* __restrict__ smark,
     int* __restrict__ stage_q, int* __restrict__ stage_w, int* __restrict__ err) {
-    __shared__ int row;
     const int t = blockIdx.x;
     if (t >= n_big) return;
     const int task = big_tasks[t];
-    if (threadIdx.x == 0) row = row_of_chunk(task_prefix, count, task);
-    __syncthreads();
-    const int ri = row;
+    const int ri = trow[task];
     const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
     const int b = task - task_prefix[ri];
     const int r0 = posT[ct_base[ri] + (long long)b * nch];
@@ -751,26 +744,17 @@ __global__ __launch_bounds__(1024) void gmerge_global(
 
 __global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
                                 const int* __restrict__ chunk_prefix,
-                                const int* __restrict__ task_prefix, int count,
+                                const int* __restrict__ task_prefix,
+                                const int* __restrict__ trow,
                                 const long long* __restrict__ ct_base,
                                 const int* __restrict__ posT, int* __restrict__ totals) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_big) return;
     const int task = big_tasks[t];
-    const int ri = row_of_chunk(task_prefix, count, task);
+    const int ri = trow[task];
     const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
     const int b = task - task_prefix[ri];
     totals[t] = posT[ct_base[ri] + (long long)(b + 1) * nch] - posT[ct_base[ri] + (long long)b * nch];
-}
-
-// The f range [f0, f1) of global chunk `chunk` (row ri of the list).
-__device__ inline void chunk_range(const int* __restrict__ list, const int* __restrict__ chunk_prefix,
-                                   const int* __restrict__ moff, const int* __restrict__ mpre,
-                                   int ri, int chunk, long long& f0, long long& f1) {
-    const int p = list[ri];
-    const long long fr0 = mpre[moff[p]], fr1 = mpre[moff[p + 1]];
-    f0 = fr0 + (long long)(chunk - chunk_prefix[ri]) * kGChunk;
-    f1 = min(fr1, f0 + kGChunk);
 }
 
 constexpr int kEmitThreads = 256;  // kGChunk / kEmitThreads = 16 staged slots per thread
@@ -786,16 +770,12 @@ __device__ inline unsigned marks_in(const unsigned* __restrict__ smark, long lon
 }
 
 // Staged entries of each chunk (one workgroup per chunk): marked bits of its f range.
-__global__ __launch_bounds__(kEmitThreads) void gemit_count(
-    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ moff, const int* __restrict__ mpre, const unsigned* __restrict__ smark,
-    int* __restrict__ ecnt) {
-    __shared__ int row, wsum[kEmitThreads / 64];
+__global__ __launch_bounds__(kEmitThreads) void gemit_count(const int4* __restrict__ db,
+                                                             const unsigned* __restrict__ smark,
+                                                             int* __restrict__ ecnt) {
+    __shared__ int wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
-    __syncthreads();
-    long long f0, f1;
-    chunk_range(list, chunk_prefix, moff, mpre, row, chunk, f0, f1);
+    const long long f0 = db[chunk].z, f1 = db[chunk].w;
     int c = 0;
     if (f1 > f0)
         for (long long w = (f0 >> 5) + threadIdx.x; w <= ((f1 - 1) >> 5); w += kEmitThreads)
@@ -815,21 +795,17 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_count(
 // chunks) + (rank in the chunk); the row's last chunk also writes its coarse degree. Thread t
 // takes the chunk's slots [16 t, 16 t + 16), so ranks follow f.
 __global__ __launch_bounds__(kEmitThreads) void gemit_write(
-    const int* __restrict__ list, int count, const int* __restrict__ chunk_prefix,
-    const int* __restrict__ moff, const int* __restrict__ mpre, const unsigned* __restrict__ smark,
+    const int4* __restrict__ da, const int4* __restrict__ db, const unsigned* __restrict__ smark,
     const int* __restrict__ stage_q, const int* __restrict__ stage_w, const int* __restrict__ epos,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
     int* __restrict__ tw) {
     constexpr int PER = kGChunk / kEmitThreads;
     static_assert(PER <= 32, "a thread's slots span at most two mark words");
-    __shared__ int row, wsum[kEmitThreads / 64];
+    __shared__ int wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
-    if (threadIdx.x == 0) row = row_of_chunk(chunk_prefix, count, chunk);
-    __syncthreads();
-    const int ri = row;
-    const int p = list[ri];
-    long long f0, f1;
-    chunk_range(list, chunk_prefix, moff, mpre, ri, chunk, f0, f1);
+    const int4 A = da[chunk], B = db[chunk];
+    const int p = B.x;
+    const long long f0 = B.z, f1 = B.w;
     const long long a0 = f0 + (long long)threadIdx.x * PER;
     // marks of the thread's slots [a0, min(a0 + PER, f1)), bit j = slot a0 + j
     unsigned bits = 0u;
@@ -851,7 +827,7 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_write(
     __syncthreads();
     int before = incl - mine;
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wsum[w];
-    const int first = chunk_prefix[ri];
+    const int first = chunk - A.y;  // the row's first chunk
     int at = tpos[p] + (epos[chunk] - epos[first]) + before;
     while (bits) {
         const int j = __ffs(bits) - 1;
@@ -860,7 +836,7 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_write(
         tw[at] = stage_w[a0 + j];
         ++at;
     }
-    if (threadIdx.x == 0 && chunk + 1 == chunk_prefix[ri + 1])
+    if (threadIdx.x == 0 && A.y + 1 == B.y)  // the row's last chunk
         cdeg[p] = epos[chunk + 1] - epos[first];
 }
 
@@ -1021,7 +997,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         KH_CHECK(hipMemcpyAsync(fd.data(), ofd_.p, sizeof(int) * novl, hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
         // per row: chunks of its member edges, buckets, count / offset table bases, tasks
-        std::vector<int> cpre(novl + 1), tpre(novl + 1);
+        std::vector<int> cpre(novl + 1), tpre(novl + 1), crow, trow;
         std::vector<long long> ofb(novl), ctb(novl);
         long long offm_total = 0, ct_total = 0;
         int nchunks = 0, ntasks = 0;
@@ -1036,6 +1012,8 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             ntasks += P;
             offm_total += (long long)nch * (P + 1);
             ct_total += (long long)nch * P;
+            crow.insert(crow.end(), nch, i);  // row of every chunk and of every (row, bucket) task
+            trow.insert(trow.end(), P, i);
         }
         cpre[novl] = nchunks;
         tpre[novl] = ntasks;
@@ -1051,6 +1029,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         KH_CHECK(hipMemcpyAsync(goffm_base_.p, ofb.data(), sizeof(long long) * novl,
                                 hipMemcpyHostToDevice, s));
         KH_CHECK(hipMemcpyAsync(gct_base_.p, ctb.data(), sizeof(long long) * novl,
+                                hipMemcpyHostToDevice, s));
+        gcrow_.reserve(nchunks);
+        gtrow_.reserve(ntasks);
+        KH_CHECK(hipMemcpyAsync(gcrow_.p, crow.data(), sizeof(int) * nchunks,
+                                hipMemcpyHostToDevice, s));
+        KH_CHECK(hipMemcpyAsync(gtrow_.p, trow.data(), sizeof(int) * ntasks,
                                 hipMemcpyHostToDevice, s));
         goffm_.reserve(offm_total);
         gcnt_.reserve(ct_total + 1);
@@ -1070,15 +1054,19 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         stage_q_.reserve(std::max<long long>(m2, 1));
         stage_w_.reserve(std::max<long long>(m2, 1));
         smark_.reserve(m2 / 32 + 2);
-        hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, ovl2_list_.p, novl,
-                           gchunk_prefix_.p, g.xadj, g.adjncy, g.adjwgt, map, moff_.p,
-                           members_.p, mpre_.p, goffm_base_.p, goffm_.p, gct_base_.p, gcnt_.p,
-                           gpart_.p, smark_.p, err_.p);
+        gda_.reserve(nchunks);
+        gdb_.reserve(nchunks);
+        hipLaunchKernelGGL(gchunk_desc, dim3(grid_for(nchunks, 256)), dim3(256), 0, s, gcrow_.p,
+                           nchunks, ovl2_list_.p, gchunk_prefix_.p, moff_.p, mpre_.p, gda_.p,
+                           gdb_.p);
+        hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, gda_.p, gdb_.p, g.xadj,
+                           g.adjncy, g.adjwgt, map, moff_.p, members_.p, mpre_.p, goffm_base_.p,
+                           goffm_.p, gct_base_.p, gcnt_.p, gpart_.p, smark_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
-        hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gchunk_prefix_.p, novl,
+        hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gda_.p, gdb_.p,
                            goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
-        hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(512), 0, s, gchunk_prefix_.p,
-                           gtask_prefix_.p, novl, ntasks, gct_base_.p, gpos_.p, gbuck_.p,
+        hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(kGMergeThreads), 0, s, gchunk_prefix_.p,
+                           gtask_prefix_.p, gtrow_.p, ntasks, gct_base_.p, gpos_.p, gbuck_.p,
                            smark_.p, stage_q_.p, stage_w_.p, gbig_.p, gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1088,7 +1076,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             DevBuf<int> totals;
             totals.reserve(nbig);
             hipLaunchKernelGGL(big_task_totals, dim3(grid_for(nbig, 256)), dim3(256), 0, s,
-                               gbig_.p, nbig, gchunk_prefix_.p, gtask_prefix_.p, novl,
+                               gbig_.p, nbig, gchunk_prefix_.p, gtask_prefix_.p, gtrow_.p,
                                gct_base_.p, gpos_.p, totals.p);
             std::vector<int> tot(nbig);
             KH_CHECK(hipMemcpyAsync(tot.data(), totals.p, sizeof(int) * nbig,
@@ -1116,7 +1104,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             fill_async(hv_.p, ttotal, 0, s);
             fill_async(hm_.p, ttotal, 0x7fffffff, s);
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
-                               gchunk_prefix_.p, gtask_prefix_.p, novl, gct_base_.p, gpos_.p,
+                               gchunk_prefix_.p, gtask_prefix_.p, gtrow_.p, gct_base_.p, gpos_.p,
                                gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, smark_.p,
                                stage_q_.p, stage_w_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
@@ -1125,12 +1113,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         ecnt_.reserve((long long)nchunks + 1);
         epos_.reserve((long long)nchunks + 1);
         fill_async(ecnt_.p + nchunks, 1, 0, s);
-        hipLaunchKernelGGL(gemit_count, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
-                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, smark_.p, ecnt_.p);
+        hipLaunchKernelGGL(gemit_count, dim3(nchunks), dim3(kEmitThreads), 0, s, gdb_.p,
+                           smark_.p, ecnt_.p);
         scan_i32(ecnt_.p, epos_.p, nchunks + 1, s);
-        hipLaunchKernelGGL(gemit_write, dim3(nchunks), dim3(kEmitThreads), 0, s, ovl2_list_.p,
-                           novl, gchunk_prefix_.p, moff_.p, mpre_.p, smark_.p, stage_q_.p,
-                           stage_w_.p, epos_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
+        hipLaunchKernelGGL(gemit_write, dim3(nchunks), dim3(kEmitThreads), 0, s, gda_.p, gdb_.p,
+                           smark_.p, stage_q_.p, stage_w_.p, epos_.p, tpos_.p, cdeg_.p, tq_.p,
+                           tw_.p);
     }
     scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
     int mc = 0, err = 0;

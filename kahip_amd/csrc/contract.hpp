@@ -46,6 +46,8 @@ private:
     DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, gbig_, gnbig_;
     DevBuf<long long> goffm_base_, gct_base_;
     DevBuf<int4> gpart_, gbuck_;
+    DevBuf<int> gcrow_, gtrow_;              // row of every global chunk / (row, bucket) task
+    DevBuf<int4> gda_, gdb_;                 // per-chunk descriptors (gchunk_desc)
     DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
     DevBuf<unsigned> smark_;                 // one bit per f: an entry is staged there
     DevBuf<int> ecnt_, epos_;                // staged entries per chunk and their scan
