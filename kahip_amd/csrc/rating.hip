@@ -305,124 +305,217 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const IN* __restrict__
     }
 }
 
-// ------------------------------------------------------------------ training (FP32 VALU)
-// One workgroup of 256 threads per 64 examples; forward with the same bf16 roundings as the
-// MFMA path (inputs, weights, layer-1 activations), backward in fp32 with straight-through
-// roundings. Per-workgroup gradients go to `partial` and are summed in a fixed order.
+// ------------------------------------------------------------------ training on MFMA
+// One workgroup of four waves per kTB = 64 examples; wave w owns examples 16w..16w+15 and runs
+// the forward exactly as mlp_forward_kernel (A operands built from the fp32 parameters, BF16
+// inputs, weights and layer-1 activations, fp32 accumulation). The backward runs its three
+// matrix products on v_mfma_f32_16x16x32_bf16 as well, with BF16 operands:
+//   dH1 = W2^T D2      per wave: its 16 edges, K = the 64 hidden units of layer 2; the B operand
+//                      is the wave's own D2 registers (same K permutation as layer 2's forward)
+//   dW2 = D2^T H1      wave w: hidden rows 16w..16w+15, K = the workgroup's 64 edges (LDS)
+//   dW1 = D1^T X       wave w: rows 16w..16w+15, 8 of the 16 columns used, K = 64 edges (LDS)
+// where D2 = dL/dZ2 and D1 = dL/dZ1 are rounded to BF16 as product operands; the bias and w3
+// gradients and the loss are fp32 sums over the edges (lane shuffles, then the four waves in
+// order). Per-workgroup gradients go to `partial` and are summed in a fixed order, so training
+// is deterministic.
 constexpr int kTB = 64;  // examples per workgroup
+
+__device__ inline float sum16(float x) {  // sum over the 16 lanes of a lane group (fixed tree)
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
 
 __global__ __launch_bounds__(256) void train_kernel(const float* __restrict__ feat,
                                                     const float* __restrict__ label, int batch,
                                                     const float* __restrict__ params,
                                                     float* __restrict__ partial,
                                                     float* __restrict__ loss_out) {
-    __shared__ float X[kTB][kFeatures];
-    __shared__ float W1[kHidden][kFeatures];
-    __shared__ float W2[kHidden][kHidden + 1];
-    __shared__ float Z1[kTB][kHidden + 1];
-    __shared__ float H1[kTB][kHidden + 1];
-    __shared__ float Z2[kTB][kHidden + 1];
-    __shared__ float D2[kTB][kHidden + 1];
-    __shared__ float dz3[kTB];
-    __shared__ float lsum[kTB];
-    const int tid = threadIdx.x;
-    const int e0 = blockIdx.x * kTB;
-    const int nb = min(kTB, batch - e0);
-    for (int i = tid; i < kTB * kFeatures; i += 256) {
-        const int e = i / kFeatures, f = i % kFeatures;
-        X[e][f] = e < nb ? round_bf(feat[(long long)(e0 + e) * kFeatures + f]) : 0.f;
-    }
-    for (int i = tid; i < kHidden * kFeatures; i += 256)
-        W1[i / kFeatures][i % kFeatures] = round_bf(params[kOffW1 + i]);
-    for (int i = tid; i < kHidden * kHidden; i += 256)
-        W2[i / kHidden][i % kHidden] = round_bf(params[kOffW2 + i]);
-    __syncthreads();
-    // layer 1
-    for (int i = tid; i < kTB * kHidden; i += 256) {
-        const int e = i / kHidden, h = i % kHidden;
-        float z = params[kOffB1 + h];
-        for (int f = 0; f < kFeatures; ++f) z += W1[h][f] * X[e][f];
-        Z1[e][h] = z;
-        H1[e][h] = round_bf(fmaxf(z, 0.f));
-    }
-    __syncthreads();
-    // layer 2
-    for (int i = tid; i < kTB * kHidden; i += 256) {
-        const int e = i / kHidden, h = i % kHidden;
-        float z = params[kOffB2 + h];
-        for (int k = 0; k < kHidden; ++k) z += W2[h][k] * H1[e][k];
-        Z2[e][h] = z;
-    }
-    __syncthreads();
-    // layer 3, loss, dz3
-    if (tid < kTB) {
-        const int e = tid;
-        float z = params[kOffB3];
-        for (int h = 0; h < kHidden; ++h) z += params[kOffW3 + h] * fmaxf(Z2[e][h], 0.f);
-        float l = 0.f, d = 0.f;
-        if (e < nb) {
-            const float y = label[e0 + e];
-            // BCE with logits, stable form
-            l = fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z)));
-            d = (1.f / (1.f + expf(-z)) - y) / (float)batch;
+    __shared__ __bf16 Xs[kTB][16];          // features (columns 8..15 zero)
+    __shared__ __bf16 H1s[kTB][kHidden];    // layer-1 activations
+    __shared__ __bf16 D2s[kTB][kHidden];    // dL/dZ2
+    __shared__ __bf16 D1s[kTB][kHidden];    // dL/dZ1
+    __shared__ float red[4][3 * kHidden + 2];  // per-wave sums: dW3, db2, db1, db3, loss
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, col = lane & 15;
+    const int le = wave * 16 + col;  // the lane's example within the workgroup
+    const long long e = (long long)blockIdx.x * kTB + le;
+    const bool valid = e < batch;
+    // A operands of the forward (as the packed weights of mlp_forward_kernel)
+    bf16x8 a1[4], a2[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            a1[t][j] = (__bf16)(k < kFeatures ? params[kOffW1 + (16 * t + col) * kFeatures + k] : 0.f);
         }
-        dz3[e] = d;
-        lsum[e] = l;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                a2[t][kb][j] = (__bf16)params[kOffW2 + (16 * t + col) * kHidden + hid_of(kb, g, j)];
     }
-    __syncthreads();
-    // dZ2 = dz3 * w3 * (Z2 > 0)
-    for (int i = tid; i < kTB * kHidden; i += 256) {
-        const int e = i / kHidden, h = i % kHidden;
-        D2[e][h] = Z2[e][h] > 0.f ? dz3[e] * params[kOffW3 + h] : 0.f;
+    // forward, layer 1: B = the 8 features of edge `col` in lane group 0
+    bf16x8 xb = {};
+    if (g == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xb[j] = (__bf16)(valid ? feat[e * kFeatures + j] : 0.f);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) Xs[le][j] = j < 8 ? xb[j] : (__bf16)0.f;
+    }
+    f32x4 z1[4];  // Z1[16t + 4g + r][col]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], xb, z, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            z1[t][r] = z[r] + params[kOffB1 + 16 * t + 4 * g + r];
+            H1s[le][16 * t + 4 * g + r] = (__bf16)fmaxf(z1[t][r], 0.f);
+        }
+    }
+    // layer 2 (B operand: the lane's own layer-1 activations in the K order of hid_of)
+    f32x4 z2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) z2[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        bf16x8 hb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) hb[j] = (__bf16)fmaxf(z1[2 * kb + (j >> 2)][j & 3], 0.f);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            z2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t][kb], hb, z2[t], 0, 0, 0);
+    }
+    float w3[4][4];
+    float acc = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int h = 16 * t + 4 * g + r;
+            z2[t][r] += params[kOffB2 + h];
+            w3[t][r] = params[kOffW3 + h];
+            acc += w3[t][r] * fmaxf(z2[t][r], 0.f);
+        }
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    const float z3 = acc + params[kOffB3];
+    // loss and dL/dz3 of the lane's edge (every lane group holds the same values)
+    float d = 0.f, l = 0.f;
+    if (valid) {
+        const float y = label[e];
+        l = fmaxf(z3, 0.f) - z3 * y + log1pf(expf(-fabsf(z3)));  // BCE with logits, stable form
+        d = (1.f / (1.f + expf(-z3)) - y) / (float)batch;
+    }
+    // D2 = d * w3 * (Z2 > 0); fp32 sums over the wave's edges for dW3 and db2
+    f32x4 D2[4];
+    float sw3[4][4], sb2[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            D2[t][r] = z2[t][r] > 0.f ? d * w3[t][r] : 0.f;
+            D2s[le][16 * t + 4 * g + r] = (__bf16)D2[t][r];
+            sw3[t][r] = sum16(d * fmaxf(z2[t][r], 0.f));
+            sb2[t][r] = sum16(D2[t][r]);
+        }
+    // dH1 = W2^T D2 (M = 64 inputs of layer 2 in 4 tiles, N = the wave's 16 edges, K = 64)
+    f32x4 D1[4];
+#pragma unroll
+    for (int tk = 0; tk < 4; ++tk) D1[tk] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        bf16x8 db;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) db[j] = (__bf16)D2[2 * kb + (j >> 2)][j & 3];
+#pragma unroll
+        for (int tk = 0; tk < 4; ++tk) {
+            bf16x8 aw;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                aw[j] = (__bf16)params[kOffW2 + hid_of(kb, g, j) * kHidden + 16 * tk + col];
+            D1[tk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw, db, D1[tk], 0, 0, 0);
+        }
+    }
+    float sb1[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            D1[t][r] = z1[t][r] > 0.f ? D1[t][r] : 0.f;
+            D1s[le][16 * t + 4 * g + r] = (__bf16)D1[t][r];
+            sb1[t][r] = sum16(D1[t][r]);
+        }
+    const float sd = sum16(d), sl = sum16(l);
+    if (col == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = 16 * t + 4 * g + r;
+                red[wave][h] = sw3[t][r];
+                red[wave][kHidden + h] = sb2[t][r];
+                red[wave][2 * kHidden + h] = sb1[t][r];
+            }
+        if (g == 0) {
+            red[wave][3 * kHidden] = sd;
+            red[wave][3 * kHidden + 1] = sl;
+        }
     }
     __syncthreads();
     float* out = partial + (long long)blockIdx.x * kMlpParams;
-    // dW3, db3
-    if (tid < kHidden) {
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += dz3[e] * fmaxf(Z2[e][tid], 0.f);
-        out[kOffW3 + tid] = s;
+    // dW2 rows 16 wave .. + 15 (A = D2^T from LDS, B = H1 from LDS, K = 64 edges)
+#pragma unroll
+    for (int tk = 0; tk < 4; ++tk) {
+        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            bf16x8 a, b;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int ee = 32 * s2 + 8 * g + j;
+                a[j] = D2s[ee][16 * wave + col];
+                b[j] = H1s[ee][16 * tk + col];
+            }
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            out[kOffW2 + (16 * wave + 4 * g + r) * kHidden + 16 * tk + col] = c[r];
     }
-    if (tid == 64) {
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += dz3[e];
-        out[kOffB3] = s;
-        float l = 0.f;
-        for (int e = 0; e < kTB; ++e) l += lsum[e];
-        loss_out[blockIdx.x] = l;
+    // dW1 rows 16 wave .. + 15 (B = X from LDS, columns 8..15 zero)
+    {
+        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            bf16x8 a, b;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int ee = 32 * s2 + 8 * g + j;
+                a[j] = D1s[ee][16 * wave + col];
+                b[j] = Xs[ee][col];
+            }
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+        }
+        if (col < kFeatures)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[kOffW1 + (16 * wave + 4 * g + r) * kFeatures + col] = c[r];
     }
-    // dW2, db2
-    for (int i = tid; i < kHidden * kHidden; i += 256) {
-        const int h = i / kHidden, k = i % kHidden;
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += D2[e][h] * H1[e][k];
-        out[kOffW2 + i] = s;
-    }
-    if (tid < kHidden) {
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += D2[e][tid];
-        out[kOffB2 + tid] = s;
-    }
-    __syncthreads();
-    // dZ1 = (W2^T dZ2) * (Z1 > 0), reusing Z1 as storage
-    for (int i = tid; i < kTB * kHidden; i += 256) {
-        const int e = i / kHidden, k = i % kHidden;
-        float s = 0.f;
-        for (int h = 0; h < kHidden; ++h) s += W2[h][k] * D2[e][h];
-        H1[e][k] = Z1[e][k] > 0.f ? s : 0.f;
-    }
-    __syncthreads();
-    // dW1, db1
-    for (int i = tid; i < kHidden * kFeatures; i += 256) {
-        const int h = i / kFeatures, f = i % kFeatures;
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += H1[e][h] * X[e][f];
-        out[kOffW1 + i] = s;
-    }
-    if (tid < kHidden) {
-        float s = 0.f;
-        for (int e = 0; e < kTB; ++e) s += H1[e][tid];
-        out[kOffB1 + tid] = s;
+    // fp32 gradients of w3, b2, b1, b3 and the loss: the four waves' sums in order
+    if (threadIdx.x < kHidden) {
+        const int h = threadIdx.x;
+        out[kOffW3 + h] = ((red[0][h] + red[1][h]) + red[2][h]) + red[3][h];
+        out[kOffB2 + h] = ((red[0][kHidden + h] + red[1][kHidden + h]) + red[2][kHidden + h]) +
+                          red[3][kHidden + h];
+        out[kOffB1 + h] = ((red[0][2 * kHidden + h] + red[1][2 * kHidden + h]) +
+                           red[2][2 * kHidden + h]) + red[3][2 * kHidden + h];
+    } else if (threadIdx.x == kHidden) {
+        out[kOffB3] = ((red[0][3 * kHidden] + red[1][3 * kHidden]) + red[2][3 * kHidden]) +
+                      red[3][3 * kHidden];
+        loss_out[blockIdx.x] = ((red[0][3 * kHidden + 1] + red[1][3 * kHidden + 1]) +
+                                red[2][3 * kHidden + 1]) + red[3][3 * kHidden + 1];
     }
 }
 

@@ -178,9 +178,12 @@ def mlp_forward(features, params, bf16=False):
     return _forward(features, params, bf16)[-1]
 
 
-def mlp_loss_grad(features, labels, params, bf16=False):
+def mlp_loss_grad(features, labels, params, bf16=False, bf16_backward=False):
     """Mean binary cross-entropy with logits and its gradient (float64). With bf16=True the
-    forward uses the GPU's roundings and the backward treats them as the identity."""
+    forward uses the GPU's roundings and the backward treats them as the identity. With
+    bf16_backward=True the backward products also take BF16 operands, as the GPU's MFMA training
+    kernel does: D2 = dL/dZ2 is rounded before dW2 = D2^T H1 and dH1 = D2 W2, D1 = dL/dZ1 before
+    dW1 = D1^T X; the bias and w3 gradients stay unrounded sums."""
     X, W1, W2, Z1, H1, Z2, H2, z = _forward(features, params, bf16)
     y = np.asarray(labels, np.float64)
     N = len(y)
@@ -191,9 +194,11 @@ def mlp_loss_grad(features, labels, params, bf16=False):
     g[OFF_W3:OFF_B3] = H2.T @ dz
     g[OFF_B3] = dz.sum()
     D2 = np.outer(dz, w3) * (Z2 > 0)
-    g[OFF_W2:OFF_B2] = (D2.T @ H1).ravel()
+    D2p = bf16_round(D2) if bf16_backward else D2
+    g[OFF_W2:OFF_B2] = (D2p.T @ H1).ravel()
     g[OFF_B2:OFF_W3] = D2.sum(axis=0)
-    D1 = (D2 @ W2) * (Z1 > 0)
-    g[OFF_W1:OFF_B1] = (D1.T @ X).ravel()
+    D1 = (D2p @ W2) * (Z1 > 0)
+    D1p = bf16_round(D1) if bf16_backward else D1
+    g[OFF_W1:OFF_B1] = (D1p.T @ X).ravel()
     g[OFF_B1:OFF_W2] = D1.sum(axis=0)
     return float(loss), g

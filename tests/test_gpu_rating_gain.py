@@ -2,8 +2,10 @@
 workload (seeds 0..9, disjoint from the training seeds 1000+), partitioned with the plain
 clustering and with the learned rating (the committed weights). The partitions are deterministic,
 so the measured geometric mean of cut(learned) / cut(plain) is a fixed number for these weights
-(0.990 at 10 clustering rounds, 0.991 at 5 when they were trained); the test requires a gain,
-i.e. a ratio below 1, at both round counts."""
+(0.9965 at 10 clustering rounds, 0.9875 at 6, when they were trained); the test requires a gain,
+i.e. a ratio below 1, at both round counts, and at 6 rounds a mean GPU time (coarsening +
+refinement + cut, rating included) no larger than the plain form's (measured 5.36 vs 5.74 ms;
+5 % allowed for timing noise)."""
 import math
 
 import numpy as np
@@ -17,11 +19,13 @@ pytestmark = pytest.mark.gpu
 
 def test_learned_rating_lowers_the_cut():
     w = K.load_rating_weights()
-    res = rating_eval.evaluate(32768, 2, 10, w, log=lambda s: None, learned_rounds=(5,))
+    res = rating_eval.evaluate(32768, 2, 10, w, log=lambda s: None, learned_rounds=(6,))
     print(f"geomean cut ratio learned / plain = {res['geomean_cut_ratio_learned_vs_plain']:.4f}; "
           f"mean GPU ms {res['mean_gpu_ms']}")
     for r in res["rows"]:
         assert r["learned"]["balance"] <= 1.03 + 1e-9
         assert r["plain"]["balance"] <= 1.03 + 1e-9
     assert res["geomean_cut_ratio_learned_vs_plain"] < 1.0
-    assert res["geomean_cut_ratio_vs_plain"]["learned_5_rounds"] < 1.0
+    assert res["geomean_cut_ratio_vs_plain"]["learned_6_rounds"] < 1.0
+    ms = res["mean_gpu_ms"]
+    assert ms["learned_6_rounds"] <= 1.05 * ms["plain"], ms

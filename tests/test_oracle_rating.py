@@ -174,3 +174,21 @@ def test_score_shift_bounds():
         s = R.score_shift(d)
         assert d * (4096 >> s) < 2 ** 31
         assert s == 0 or d * (4096 >> (s - 1)) >= 2 ** 31
+
+
+def test_bf16_backward_rounds_only_the_product_operands():
+    """bf16_backward changes the gradients that come from the rounded D2 / D1 (dW2, dW1 and,
+    through D1 = D2 W2, db1) by the BF16 operand error (relative 2^-9 per operand) and leaves
+    db2, dw3, db3 and the loss unchanged."""
+    rng = np.random.default_rng(3)
+    X = np.abs(rng.normal(size=(200, 8))) * 2
+    y = (rng.random(200) < 0.4).astype(np.float64)
+    p = rng.normal(size=R.PARAMS) * 0.3
+    l0, g0 = R.mlp_loss_grad(X, y, p, bf16=True)
+    l1, g1 = R.mlp_loss_grad(X, y, p, bf16=True, bf16_backward=True)
+    assert l0 == l1
+    for lo, hi in ((R.OFF_B2, R.OFF_W3), (R.OFF_W3, R.PARAMS)):
+        assert np.array_equal(g0[lo:hi], g1[lo:hi])
+    for lo, hi in ((R.OFF_W1, R.OFF_B1), (R.OFF_B1, R.OFF_W2), (R.OFF_W2, R.OFF_B2)):
+        d = np.linalg.norm(g1[lo:hi] - g0[lo:hi]) / np.linalg.norm(g0[lo:hi])
+        assert 0 < d < 2.0 ** -8
