@@ -447,10 +447,10 @@ __global__ void member_degrees(int n, const int* __restrict__ members,
 // P_i = min(chunks, kGMaxBuckets) buckets of neighbour clusters q (bucket = gbucket(q, P_i)).
 //   gchunk   each chunk aggregates (q -> sum w, min f) in LDS and writes its entries grouped by
 //            bucket (per-chunk bucket offsets offm, bucket-major counts cntT); it also clears
-//            the stage slots of its f range;
-//   gscatter after a scan of cntT, moves them so that every (row, bucket) is one range;
-//   gmerge   one workgroup per (row, bucket) sums the range in LDS and stores every merged
-//            entry (q, w) in the stage at f = its min f; ranges of more than kGMergeMax entries
+//            the stage marks of its f range;
+//   gmerge   one workgroup per (row, bucket) reads the bucket's segment of every chunk of the
+//            row (located through the scan of cntT), sums them in LDS and stores every merged
+//            entry (q, w) in the stage at f = its min f; tasks of more than kGMergeMax entries
 //            are queued for gmerge_global (a private global table per task);
 //   gemit    counts the staged entries of each chunk, then (after a scan over the chunks)
 //            writes them in f order, which is the order of the first appearance of q in the
@@ -629,47 +629,68 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     }
 }
 
-__global__ __launch_bounds__(256) void gscatter(
-    const int4* __restrict__ da, const int4* __restrict__ db,
-    const long long* __restrict__ offm_base, const int* __restrict__ offm,
-    const long long* __restrict__ ct_base, const int* __restrict__ posT,
-    const int4* __restrict__ partials, int4* __restrict__ bucketed) {
-    const int chunk = blockIdx.x;
-    const int ri = da[chunk].x, c_idx = da[chunk].y;
-    const int nch = db[chunk].y;
-    const int P = min(nch, kGMaxBuckets);
-    const int* offs = offm + offm_base[ri] + (long long)c_idx * (P + 1);
-    const int* pos = posT + ct_base[ri] + c_idx;
-    const int4* src = partials + (long long)chunk * kGChunk;
-    const int total = offs[P];
-    for (int i = threadIdx.x; i < total; i += 256) {
-        const int4 e = src[i];
-        const int b = gbucket(e.x, P);
-        bucketed[pos[(long long)b * nch] + (i - offs[b])] = e;
+// Entry i of merge task (row ri, bucket b), read where its chunk wrote it (no scatter pass):
+// chunk c of the row holds entries [pre(c), pre(c + 1)) of the task, pre(c) = posT[base + c] -
+// r0 with base = ct_base[ri] + b * nch (posT: exclusive scan of the bucket-major counts), at
+// partials[(c0 + c) * kGChunk + start(c) + ...], start(c) = offm[offm_base[ri] + c (P + 1) + b].
+// The chunk of entry i is the largest c with pre(c) <= i (empty segments are skipped).
+constexpr int kGStage = 1024;  // rows of at most this many chunks stage pre / start in LDS
+struct TaskSegs {
+    const int* posT;  // + base
+    const int* offs;  // offm + offm_base[ri] + b, stride P + 1
+    long long stride;
+    int r0, nch;
+    long long c0;
+};
+template <bool STAGED>
+__device__ inline int4 task_entry(const TaskSegs& t, const int* spre, const int* sstart,
+                                  const int4* __restrict__ partials, int i) {
+    int lo = 0, hi = t.nch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int pm = STAGED ? spre[mid] : t.posT[mid] - t.r0;
+        if (pm <= i) lo = mid; else hi = mid - 1;
     }
+    const int pre = STAGED ? spre[lo] : t.posT[lo] - t.r0;
+    const int start = STAGED ? sstart[lo] : t.offs[(long long)lo * t.stride];
+    return partials[(t.c0 + lo) * kGChunk + start + (i - pre)];
 }
 
 __global__ __launch_bounds__(kGMergeThreads) void gmerge(
     const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix,
     const int* __restrict__ trow, int n_tasks, const long long* __restrict__ ct_base,
-    const int* __restrict__ posT, const int4* __restrict__ bucketed, unsigned* __restrict__ smark, int* __restrict__ stage_q,
-    int* __restrict__ stage_w, int* __restrict__ big_tasks, int* __restrict__ n_big,
-    int* __restrict__ err) {
+    const int* __restrict__ posT, const long long* __restrict__ offm_base,
+    const int* __restrict__ offm, const int4* __restrict__ partials,
+    unsigned* __restrict__ smark, int* __restrict__ stage_q, int* __restrict__ stage_w,
+    int* __restrict__ big_tasks, int* __restrict__ n_big, int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
+    __shared__ int spre[kGStage + 1];
+    __shared__ int sstart[kGStage];
     const int task = blockIdx.x;
     if (task >= n_tasks) return;  // uniform
     const int ri = trow[task];
-    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    TaskSegs ts;
+    ts.c0 = chunk_prefix[ri];
+    ts.nch = chunk_prefix[ri + 1] - (int)ts.c0;
     const int b = task - task_prefix[ri];
-    const int r0 = posT[ct_base[ri] + (long long)b * nch];
-    const int r1 = posT[ct_base[ri] + (long long)(b + 1) * nch];
-    const int total = r1 - r0;
+    const int P = min(ts.nch, kGMaxBuckets);
+    ts.posT = posT + ct_base[ri] + (long long)b * ts.nch;
+    ts.offs = offm + offm_base[ri] + b;
+    ts.stride = P + 1;
+    ts.r0 = ts.posT[0];
+    const int total = ts.posT[ts.nch] - ts.r0;
     if (total > kGMergeMax) {
         if (threadIdx.x == 0) big_tasks[atomicAdd(n_big, 1)] = task;
         return;  // uniform
     }
+    const bool staged = ts.nch <= kGStage;  // uniform
+    if (staged)
+        for (int c = threadIdx.x; c <= ts.nch; c += kGMergeThreads) {
+            spre[c] = ts.posT[c] - ts.r0;
+            if (c < ts.nch) sstart[c] = ts.offs[(long long)c * ts.stride];
+        }
     int cap = 64;
     while (cap < 2 * total) cap <<= 1;
     for (int i = threadIdx.x; i < cap; i += kGMergeThreads) {
@@ -678,9 +699,10 @@ __global__ __launch_bounds__(kGMergeThreads) void gmerge(
         M[i] = 0x7fffffff;
     }
     __syncthreads();
-    for (int i = r0 + threadIdx.x; i < r1; i += kGMergeThreads) {
-        const int4 e = bucketed[i];
-        const int h = probe_claim(K, cap, e.x);  // cap >= 2 x the range's entries
+    for (int i = threadIdx.x; i < total; i += kGMergeThreads) {
+        const int4 e = staged ? task_entry<true>(ts, spre, sstart, partials, i)
+                              : task_entry<false>(ts, spre, sstart, partials, i);
+        const int h = probe_claim(K, cap, e.x);  // cap >= 2 x the task's entries
         if (h < 0) {
             *err = kErrProbe;
             continue;
@@ -705,24 +727,32 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     const int* __restrict__ big_tasks, int n_big, const int* __restrict__ chunk_prefix,
     const int* __restrict__ task_prefix, const int* __restrict__ trow,
     const long long* __restrict__ ct_base, const int* __restrict__ posT,
-    const int4* __restrict__ bucketed, const long long* __restrict__ toff, const long long* __restrict__ tcap, int* __restrict__ TK,
-    int* __restrict__ TV, int* __restrict__ TM, unsigned* __restrict__ smark,
-    int* __restrict__ stage_q, int* __restrict__ stage_w, int* __restrict__ err) {
+    const long long* __restrict__ offm_base, const int* __restrict__ offm,
+    const int4* __restrict__ partials, const long long* __restrict__ toff,
+    const long long* __restrict__ tcap, int* __restrict__ TK, int* __restrict__ TV,
+    int* __restrict__ TM, unsigned* __restrict__ smark, int* __restrict__ stage_q,
+    int* __restrict__ stage_w, int* __restrict__ err) {
     const int t = blockIdx.x;
     if (t >= n_big) return;
     const int task = big_tasks[t];
     const int ri = trow[task];
-    const int nch = chunk_prefix[ri + 1] - chunk_prefix[ri];
+    TaskSegs ts;
+    ts.c0 = chunk_prefix[ri];
+    ts.nch = chunk_prefix[ri + 1] - (int)ts.c0;
     const int b = task - task_prefix[ri];
-    const int r0 = posT[ct_base[ri] + (long long)b * nch];
-    const int r1 = posT[ct_base[ri] + (long long)(b + 1) * nch];
+    const int P = min(ts.nch, kGMaxBuckets);
+    ts.posT = posT + ct_base[ri] + (long long)b * ts.nch;
+    ts.offs = offm + offm_base[ri] + b;
+    ts.stride = P + 1;
+    ts.r0 = ts.posT[0];
+    const int total = ts.posT[ts.nch] - ts.r0;
     const long long cap = tcap[t];
     int* K = TK + toff[t];
     int* V = TV + toff[t];
     int* Mn = TM + toff[t];
-    for (int i = r0 + threadIdx.x; i < r1; i += 1024) {
-        const int4 e = bucketed[i];
-        const long long h = probe_claim(K, cap, e.x);  // cap >= 2 x the range's entries
+    for (int i = threadIdx.x; i < total; i += 1024) {
+        const int4 e = task_entry<false>(ts, nullptr, nullptr, partials, i);
+        const long long h = probe_claim(K, cap, e.x);  // cap >= 2 x the task's entries
         if (h < 0) {
             *err = kErrProbe;
             continue;
@@ -1040,7 +1070,6 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         gcnt_.reserve(ct_total + 1);
         gpos_.reserve(ct_total + 1);
         gpart_.reserve((long long)nchunks * kGChunk);
-        gbuck_.reserve((long long)nchunks * kGChunk);
         gbig_.reserve(std::max(ntasks, 1));
         gnbig_.reserve(1);
         fill_async(gcnt_.p + ct_total, 1, 0, s);  // sentinel
@@ -1063,11 +1092,10 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
                            g.adjncy, g.adjwgt, map, moff_.p, members_.p, mpre_.p, goffm_base_.p,
                            goffm_.p, gct_base_.p, gcnt_.p, gpart_.p, smark_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
-        hipLaunchKernelGGL(gscatter, dim3(nchunks), dim3(256), 0, s, gda_.p, gdb_.p,
-                           goffm_base_.p, goffm_.p, gct_base_.p, gpos_.p, gpart_.p, gbuck_.p);
         hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(kGMergeThreads), 0, s, gchunk_prefix_.p,
-                           gtask_prefix_.p, gtrow_.p, ntasks, gct_base_.p, gpos_.p, gbuck_.p,
-                           smark_.p, stage_q_.p, stage_w_.p, gbig_.p, gnbig_.p, err_.p);
+                           gtask_prefix_.p, gtrow_.p, ntasks, gct_base_.p, gpos_.p,
+                           goffm_base_.p, goffm_.p, gpart_.p, smark_.p, stage_q_.p, stage_w_.p,
+                           gbig_.p, gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -1105,8 +1133,8 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             fill_async(hm_.p, ttotal, 0x7fffffff, s);
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
                                gchunk_prefix_.p, gtask_prefix_.p, gtrow_.p, gct_base_.p, gpos_.p,
-                               gbuck_.p, hoff_.p, hcap_.p, hk_.p, hv_.p, hm_.p, smark_.p,
-                               stage_q_.p, stage_w_.p, err_.p);
+                               goffm_base_.p, goffm_.p, gpart_.p, hoff_.p, hcap_.p, hk_.p, hv_.p,
+                               hm_.p, smark_.p, stage_q_.p, stage_w_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
         }
         // emission in f order: counts per chunk, a scan over the chunks, ordered writes

@@ -45,7 +45,7 @@ private:
     DevBuf<int> hk_, hv_, hm_;
     DevBuf<int> gtask_prefix_, goffm_, gcnt_, gpos_, gbig_, gnbig_;
     DevBuf<long long> goffm_base_, gct_base_;
-    DevBuf<int4> gpart_, gbuck_;
+    DevBuf<int4> gpart_;
     DevBuf<int> gcrow_, gtrow_;              // row of every global chunk / (row, bucket) task
     DevBuf<int4> gda_, gdb_;                 // per-chunk descriptors (gchunk_desc)
     DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
