@@ -58,6 +58,8 @@ constexpr int kTinyMax = 1024;
 constexpr int kSmallThreads = 512;   // small bucket merge: <= kSmallMax entries in 32 KB of LDS
 constexpr int kSmallCap = 4096;
 constexpr int kSmallMax = 2048;
+constexpr int kHubStage = 256;       // merge tasks of hubs of at most this many chunks stage the
+                                     // chunks' segment offsets in LDS
 constexpr int kBigDegree = 64;    // moved nodes above this degree mark neighbours per workgroup
 
 struct Cand {
@@ -739,42 +741,6 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
     }
 }
 
-// Moves the bucket-grouped entries of every chunk of a multi-chunk hub to their bucket-major
-// position (posT = exclusive scan of the bucket-major counts), so that every bucket of a hub is
-// one contiguous range.
-__global__ __launch_bounds__(256) void hub_scatter(
-    const int4* __restrict__ cmap, const long long* __restrict__ offm_base,
-    const int* __restrict__ offm, const int* __restrict__ ct_base, const int* __restrict__ posT,
-    const int2* __restrict__ partials, int2* __restrict__ bucketed) {
-    const int chunk = blockIdx.x;
-    const int4 cm = cmap[chunk];
-    const int gi = cm.x;
-    const int nch = cm.z;
-    if (nch <= 1) return;
-    const int c_idx = cm.y;
-    const int P = min(kBucketsPerChunk * nch, kMaxBuckets);
-    const int* offs = offm + offm_base[gi] + (long long)c_idx * (P + 1);
-    const int* pos = posT + ct_base[gi] + c_idx;
-    const int2* src = partials + (long long)chunk * kHubChunk;
-    // destination of entry i of bucket b: pos[b * nch] + (i - offs[b]), staged per bucket
-    __shared__ int sbase[kMaxBuckets];
-    for (int b = threadIdx.x; b < P; b += 256) sbase[b] = pos[(long long)b * nch] - offs[b];
-    const int total = offs[P];
-    __syncthreads();
-    constexpr int U = 4;  // entries per thread in flight
-    for (int i0 = threadIdx.x; i0 < total; i0 += 256 * U) {
-        int2 kv[U];
-#pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int i = i0 + 256 * j;
-            kv[j] = i < total ? src[i] : make_int2(-1, 0);
-        }
-#pragma unroll
-        for (int j = 0; j < U; ++j)
-            if (kv[j].x >= 0) bucketed[sbase[bucket_of(kv[j].x, P)] + i0 + 256 * j] = kv[j];
-    }
-}
-
 // Merge of one (multi-chunk hub, bucket) task: the bucket's entries of all chunks of the hub are
 // summed in an LDS hash (or, past kFinLds entries, in the hub's global table, filtering by
 // bucket); the bucket's best candidate and, for the bucket of the own label, conn(own) are left
@@ -786,8 +752,9 @@ template <LpMode M, int LEVEL>
 __global__ __launch_bounds__(LEVEL == 0 ? kTinyThreads : (LEVEL == 1 ? kSmallThreads : kFinThreads))
 void hub_bucket_merge(
     LpArgs a, int n_tasks, const int4* __restrict__ tmap, const int2* __restrict__ tsub,
-    const int* __restrict__ posT, const int2* __restrict__ bucketed,
-    const long long* __restrict__ tab_off,
+    const int* __restrict__ posT, const int* __restrict__ chunk_prefix,
+    const long long* __restrict__ offm_base, const int* __restrict__ offm,
+    const int2* __restrict__ partials, const long long* __restrict__ tab_off,
     int* __restrict__ gkey, int* __restrict__ gval, Cand* __restrict__ bres,
     int* __restrict__ bown, uint8_t* __restrict__ fb, const int* __restrict__ in_q,
     const int* __restrict__ in_n, int* __restrict__ out_q, int* __restrict__ out_n) {
@@ -797,6 +764,8 @@ void hub_bucket_merge(
     static_assert(2 * MAXE <= CAP && 2 * kFinLds <= kFinCap, "merge tables fill at most half");
     __shared__ int K[CAP];
     __shared__ int V[CAP];
+    __shared__ int spre[kHubStage + 1];  // task offsets of the chunks' bucket segments
+    __shared__ int sstart[kHubStage];    // the segments' starts in their chunks' partials
     const int n_iter = LEVEL == 0 ? n_tasks : *in_n;
     for (int it = blockIdx.x; it < n_iter; it += gridDim.x) {
         const int task = LEVEL == 0 ? it : in_q[it];
@@ -821,6 +790,19 @@ void hub_bucket_merge(
         }
         const bool in_lds = LEVEL < 2 || total <= kFinLds;
         const int deg = a.xadj[v + 1] - a.xadj[v];
+        // The task's entries are read where the hub's chunks wrote them: chunk c holds entries
+        // [pre(c), pre(c + 1)) of the task, pre(c) = posT[tm.x + c] - r0, in its partials at
+        // offm[offm_base[gi] + c (P + 1) + b]; the chunk of entry i is the largest c with
+        // pre(c) <= i (empty segments are skipped).
+        const int nch = tm.y - tm.x;
+        const long long c0 = chunk_prefix[gi];
+        const int* offs = offm + offm_base[gi] + b;
+        const bool staged = nch <= kHubStage;  // uniform
+        if (staged)
+            for (int c = threadIdx.x; c <= nch; c += THREADS) {
+                spre[c] = posT[tm.x + c] - r0;
+                if (c < nch) sstart[c] = offs[(long long)c * (P + 1)];
+            }
         long long cap = CAP;
         int* TK = K;
         int* TV = V;
@@ -839,8 +821,16 @@ void hub_bucket_merge(
             if (threadIdx.x == 0) fb[mi] = 1;
         }
         __syncthreads();
-        for (int i = r0 + threadIdx.x; i < r1; i += THREADS) {
-            const int2 kv = bucketed[i];
+        for (int i = threadIdx.x; i < total; i += THREADS) {
+            int lo = 0, hi = nch - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                const int pm = staged ? spre[mid] : posT[tm.x + mid] - r0;
+                if (pm <= i) lo = mid; else hi = mid - 1;
+            }
+            const int pre = staged ? spre[lo] : posT[tm.x + lo] - r0;
+            const int start = staged ? sstart[lo] : offs[(long long)lo * (P + 1)];
+            const int2 kv = partials[(c0 + lo) * kHubChunk + start + (i - pre)];
             // LDS: cap >= 2 x the bucket's entries; global: cap >= 2 x the hub's degree
             const long long h = probe_claim(TK, cap, kv.x);
             if (h < 0) {
@@ -1583,7 +1573,6 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     int max_chunks = 0;
     for (int sr = 0; sr < S; ++sr) max_chunks = std::max(max_chunks, hub_chunks_[sr]);
     partials_.reserve((long long)max_chunks * kHubChunk);
-    bucketed_.reserve((long long)max_chunks * kHubChunk);
     KH_CHECK(hipStreamSynchronize(s));  // the host vectors above must outlive the copies
     KH_CHECK(hipGetLastError());
 }
@@ -1760,20 +1749,20 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                             size_t sb = scan_tmp_bytes_;
                             KH_CHECK(rocprim::exclusive_scan(scan_tmp_.p, sb, cnt_t_.p + ct_sr_base_[sr], pos_t_.p + ct_sr_base_[sr], 0, (size_t)ct_sr_size_[sr] + 1, rocprim::plus<int>(), s));
                             const int* pt = pos_t_.p;
-                            hipLaunchKernelGGL(hub_scatter, dim3(hub_chunks_[sr]), dim3(256), 0, s, cm, ob, offm_.p, ctb, pt, partials_.p, bucketed_.p);
+                            const int* cp = chunk_prefix_.p + chunk_base_[sr];
                             int* bt = big_tasks_.p;
                             int* nbt = n_big_tasks_.p + (r * S + sr);
                             int* mt = mid_tasks_.p;
                             int* nmt = n_mid_tasks_.p + (r * S + sr);
                             if (cl) {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Cluster, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Cluster>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             } else {
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
-                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, bucketed_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 0>), dim3(nt), dim3(kTinyThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, nullptr, nullptr, mt, nmt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 1>), dim3(std::min(nt, 1024)), dim3(kSmallThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, mt, nmt, bt, nbt);
+                                hipLaunchKernelGGL((hub_bucket_merge<LpMode::Refine, 2>), dim3(std::min(nt, 256)), dim3(kFinThreads), 0, s, a, nt, tm, tsb, pt, cp, ob, offm_.p, partials_.p, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p, bt, nbt, nullptr, nullptr);
                                 hipLaunchKernelGGL((hub_decide<LpMode::Refine>), dim3(nm), dim3(256), 0, s, a, lst, ml, nm, tp, to, hub_key_.p, hub_val_.p, br, bown_.p, fb_.p);
                             }
                         }

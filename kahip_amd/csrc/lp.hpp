@@ -100,7 +100,7 @@ private:
     size_t scan_tmp_bytes_ = 0;
     DevBuf<int> hub_deg_, chunk_prefix_, multi_, big_, n_big_;
     DevBuf<long long> tab_off_, tab_cap_;
-    DevBuf<int2> partials_, bucketed_;
+    DevBuf<int2> partials_;
     DevBuf<int> ct_base_, cnt_t_, pos_t_;
     DevBuf<char> scan_tmp_;
     DevBuf<long long> offm_base_;
