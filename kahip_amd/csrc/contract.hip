@@ -447,7 +447,7 @@ __global__ void member_degrees(int n, const int* __restrict__ members,
 // P_i = min(chunks, kGMaxBuckets) buckets of neighbour clusters q (bucket = gbucket(q, P_i)).
 //   gchunk   each chunk aggregates (q -> sum w, min f) in LDS and writes its entries grouped by
 //            bucket (per-chunk bucket offsets offm, bucket-major counts cntT); it also clears
-//            the stage marks of its f range;
+//            the stage slots of its f range;
 //   gmerge   one workgroup per (row, bucket) reads the bucket's segment of every chunk of the
 //            row (located through the scan of cntT), sums them in LDS and stores every merged
 //            entry (q, w) in the stage at f = its min f; tasks of more than kGMergeMax entries
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     const int* __restrict__ moff, const int* __restrict__ members, const int* __restrict__ mpre,
     const long long* __restrict__ offm_base, int* __restrict__ offm,
     const long long* __restrict__ ct_base, int* __restrict__ cntT, int4* __restrict__ partials,
-    unsigned* __restrict__ smark, int* __restrict__ err) {
+    int* __restrict__ stage_q, int* __restrict__ err) {
     __shared__ int K[kGCap];
     __shared__ int V[kGCap];
     __shared__ int M[kGCap];
@@ -531,11 +531,9 @@ __global__ __launch_bounds__(kGThreads) void gchunk(
     constexpr int PER = kGChunk / kGThreads;
     const int m0 = A.z, nm = A.w;
     const bool staged = nm <= kGChunk;  // uniform
-    // clear the stage marks of [f0, f1) (words shared with a neighbouring range get the same 0;
-    // marks are set only by the merge kernels, after this one)
-    if (f1 > f0)
-        for (long long w = (f0 >> 5) + threadIdx.x; w <= ((f1 - 1) >> 5); w += kGThreads)
-            smark[w] = 0u;
+    // clear the stage of [f0, f1) (the merge kernels fill it after this one); a sequential
+    // 4-byte store per edge measured faster than a mark bitmap set with atomics by the merges
+    for (long long f = f0 + threadIdx.x; f < f1; f += kGThreads) stage_q[f] = -1;
     if (staged)
         for (int i = threadIdx.x; i < nm; i += kGThreads) {
             sbeg[i] = mpre[m0 + i];
@@ -660,9 +658,9 @@ __global__ __launch_bounds__(kGMergeThreads) void gmerge(
     const int* __restrict__ chunk_prefix, const int* __restrict__ task_prefix,
     const int* __restrict__ trow, int n_tasks, const long long* __restrict__ ct_base,
     const int* __restrict__ posT, const long long* __restrict__ offm_base,
-    const int* __restrict__ offm, const int4* __restrict__ partials,
-    unsigned* __restrict__ smark, int* __restrict__ stage_q, int* __restrict__ stage_w,
-    int* __restrict__ big_tasks, int* __restrict__ n_big, int* __restrict__ err) {
+    const int* __restrict__ offm, const int4* __restrict__ partials, int* __restrict__ stage_q,
+    int* __restrict__ stage_w, int* __restrict__ big_tasks, int* __restrict__ n_big,
+    int* __restrict__ err) {
     __shared__ int K[kGMergeCap];
     __shared__ int V[kGMergeCap];
     __shared__ int M[kGMergeCap];
@@ -715,7 +713,6 @@ __global__ __launch_bounds__(kGMergeThreads) void gmerge(
         const int q = K[i];
         if (q < 0) continue;
         const int f = M[i];  // one merged entry per (row, q): its min f is unique
-        atomicOr(&smark[f >> 5], 1u << (f & 31));
         stage_q[f] = q;
         stage_w[f] = V[i];
     }
@@ -730,8 +727,8 @@ __global__ __launch_bounds__(1024) void gmerge_global(
     const long long* __restrict__ offm_base, const int* __restrict__ offm,
     const int4* __restrict__ partials, const long long* __restrict__ toff,
     const long long* __restrict__ tcap, int* __restrict__ TK, int* __restrict__ TV,
-    int* __restrict__ TM, unsigned* __restrict__ smark, int* __restrict__ stage_q,
-    int* __restrict__ stage_w, int* __restrict__ err) {
+    int* __restrict__ TM, int* __restrict__ stage_q, int* __restrict__ stage_w,
+    int* __restrict__ err) {
     const int t = blockIdx.x;
     if (t >= n_big) return;
     const int task = big_tasks[t];
@@ -766,7 +763,6 @@ __global__ __launch_bounds__(1024) void gmerge_global(
         const int q = K[i];
         if (q < 0) continue;
         const int f = Mn[i];
-        atomicOr(&smark[f >> 5], 1u << (f & 31));
         stage_q[f] = q;
         stage_w[f] = V[i];
     }
@@ -790,26 +786,15 @@ __global__ void big_task_totals(const int* __restrict__ big_tasks, int n_big,
 constexpr int kEmitThreads = 256;  // kGChunk / kEmitThreads = 16 staged slots per thread
 static_assert(kGChunk % kEmitThreads == 0, "every emission thread takes whole slots");
 
-// Marked bits of [f0, f1) in word w of the stage marks.
-__device__ inline unsigned marks_in(const unsigned* __restrict__ smark, long long w, long long f0,
-                                    long long f1) {
-    const long long b0 = w << 5;
-    const int lo = (int)(max(f0, b0) - b0), hi = (int)(min(f1, b0 + 32) - b0);  // 0 <= lo < hi <= 32
-    const unsigned m = (hi == 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
-    return smark[w] & m;
-}
-
-// Staged entries of each chunk (one workgroup per chunk): marked bits of its f range.
+// Staged entries of each chunk (one workgroup per chunk).
 __global__ __launch_bounds__(kEmitThreads) void gemit_count(const int4* __restrict__ db,
-                                                             const unsigned* __restrict__ smark,
+                                                             const int* __restrict__ stage_q,
                                                              int* __restrict__ ecnt) {
     __shared__ int wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
     const long long f0 = db[chunk].z, f1 = db[chunk].w;
     int c = 0;
-    if (f1 > f0)
-        for (long long w = (f0 >> 5) + threadIdx.x; w <= ((f1 - 1) >> 5); w += kEmitThreads)
-            c += __popc(marks_in(smark, w, f0, f1));
+    for (long long f = f0 + threadIdx.x; f < f1; f += kEmitThreads) c += stage_q[f] >= 0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
@@ -825,27 +810,23 @@ __global__ __launch_bounds__(kEmitThreads) void gemit_count(const int4* __restri
 // chunks) + (rank in the chunk); the row's last chunk also writes its coarse degree. Thread t
 // takes the chunk's slots [16 t, 16 t + 16), so ranks follow f.
 __global__ __launch_bounds__(kEmitThreads) void gemit_write(
-    const int4* __restrict__ da, const int4* __restrict__ db, const unsigned* __restrict__ smark,
-    const int* __restrict__ stage_q, const int* __restrict__ stage_w, const int* __restrict__ epos,
+    const int4* __restrict__ da, const int4* __restrict__ db, const int* __restrict__ stage_q,
+    const int* __restrict__ stage_w, const int* __restrict__ epos,
     const int* __restrict__ tpos, int* __restrict__ cdeg, int* __restrict__ tq,
     int* __restrict__ tw) {
     constexpr int PER = kGChunk / kEmitThreads;
-    static_assert(PER <= 32, "a thread's slots span at most two mark words");
+    static_assert(PER <= 32, "a thread's staged slots fit one bit mask");
     __shared__ int wsum[kEmitThreads / 64];
     const int chunk = blockIdx.x;
     const int4 A = da[chunk], B = db[chunk];
     const int p = B.x;
     const long long f0 = B.z, f1 = B.w;
     const long long a0 = f0 + (long long)threadIdx.x * PER;
-    // marks of the thread's slots [a0, min(a0 + PER, f1)), bit j = slot a0 + j
+    // staged slots among the thread's [a0, min(a0 + PER, f1)), bit j = slot a0 + j
     unsigned bits = 0u;
-    if (a0 < f1) {
-        const long long a1 = min(f1, a0 + PER);
-        const long long w0 = a0 >> 5, w1 = (a1 - 1) >> 5;
-        const int sh = (int)(a0 & 31);
-        bits = marks_in(smark, w0, a0, a1) >> sh;
-        if (w1 != w0) bits |= marks_in(smark, w1, a0, a1) << (32 - sh);
-    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+        if (a0 + j < f1 && stage_q[a0 + j] >= 0) bits |= 1u << j;
     const int mine = __popc(bits);
     int incl = mine;
 #pragma unroll
@@ -1082,7 +1063,6 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         scan_i32(mdeg_.p, mpre_.p, n + 1, s);
         stage_q_.reserve(std::max<long long>(m2, 1));
         stage_w_.reserve(std::max<long long>(m2, 1));
-        smark_.reserve(m2 / 32 + 2);
         gda_.reserve(nchunks);
         gdb_.reserve(nchunks);
         hipLaunchKernelGGL(gchunk_desc, dim3(grid_for(nchunks, 256)), dim3(256), 0, s, gcrow_.p,
@@ -1090,12 +1070,12 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
                            gdb_.p);
         hipLaunchKernelGGL(gchunk, dim3(nchunks), dim3(kGThreads), 0, s, gda_.p, gdb_.p, g.xadj,
                            g.adjncy, g.adjwgt, map, moff_.p, members_.p, mpre_.p, goffm_base_.p,
-                           goffm_.p, gct_base_.p, gcnt_.p, gpart_.p, smark_.p, err_.p);
+                           goffm_.p, gct_base_.p, gcnt_.p, gpart_.p, stage_q_.p, err_.p);
         scan_i32(gcnt_.p, gpos_.p, (int)ct_total + 1, s);
         hipLaunchKernelGGL(gmerge, dim3(ntasks), dim3(kGMergeThreads), 0, s, gchunk_prefix_.p,
                            gtask_prefix_.p, gtrow_.p, ntasks, gct_base_.p, gpos_.p,
-                           goffm_base_.p, goffm_.p, gpart_.p, smark_.p, stage_q_.p, stage_w_.p,
-                           gbig_.p, gnbig_.p, err_.p);
+                           goffm_base_.p, goffm_.p, gpart_.p, stage_q_.p, stage_w_.p, gbig_.p,
+                           gnbig_.p, err_.p);
         int nbig = 0;
         KH_CHECK(hipMemcpyAsync(&nbig, gnbig_.p, sizeof(int), hipMemcpyDeviceToHost, s));
         KH_CHECK(hipStreamSynchronize(s));
@@ -1134,7 +1114,7 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
             hipLaunchKernelGGL(gmerge_global, dim3(nbig), dim3(1024), 0, s, gbig_.p, nbig,
                                gchunk_prefix_.p, gtask_prefix_.p, gtrow_.p, gct_base_.p, gpos_.p,
                                goffm_base_.p, goffm_.p, gpart_.p, hoff_.p, hcap_.p, hk_.p, hv_.p,
-                               hm_.p, smark_.p, stage_q_.p, stage_w_.p, err_.p);
+                               hm_.p, stage_q_.p, stage_w_.p, err_.p);
             KH_CHECK(hipStreamSynchronize(s));  // host vectors above must outlive the copies
         }
         // emission in f order: counts per chunk, a scan over the chunks, ordered writes
@@ -1142,11 +1122,10 @@ void ContractWorkspace::contract(const DGraph& g, const int* map, int nc, OwnedG
         epos_.reserve((long long)nchunks + 1);
         fill_async(ecnt_.p + nchunks, 1, 0, s);
         hipLaunchKernelGGL(gemit_count, dim3(nchunks), dim3(kEmitThreads), 0, s, gdb_.p,
-                           smark_.p, ecnt_.p);
+                           stage_q_.p, ecnt_.p);
         scan_i32(ecnt_.p, epos_.p, nchunks + 1, s);
         hipLaunchKernelGGL(gemit_write, dim3(nchunks), dim3(kEmitThreads), 0, s, gda_.p, gdb_.p,
-                           smark_.p, stage_q_.p, stage_w_.p, epos_.p, tpos_.p, cdeg_.p, tq_.p,
-                           tw_.p);
+                           stage_q_.p, stage_w_.p, epos_.p, tpos_.p, cdeg_.p, tq_.p, tw_.p);
     }
     scan_i32(cdeg_.p, out.xadj_dev(), nc + 1, s);
     int mc = 0, err = 0;

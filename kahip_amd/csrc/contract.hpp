@@ -49,7 +49,6 @@ private:
     DevBuf<int> gcrow_, gtrow_;              // row of every global chunk / (row, bucket) task
     DevBuf<int4> gda_, gdb_;                 // per-chunk descriptors (gchunk_desc)
     DevBuf<int> stage_q_, stage_w_;          // merged entries at the f of their first edge
-    DevBuf<unsigned> smark_;                 // one bit per f: an entry is staged there
     DevBuf<int> ecnt_, epos_;                // staged entries per chunk and their scan
     DevBuf<int> err_;  // device error flag (kErrProbe) of the last contract()
 };
