@@ -914,10 +914,7 @@ __global__ __launch_bounds__(256) void hub_decide(
 // ---------------------------------------------------------------- filter / admit / apply
 // The filter / admit / apply kernels run over the node list of the sub-round; nodes without a
 // proposal (prop_t < 0) return at once.
-__global__ void filter_departures(LpArgs a, const int* __restrict__ list, int count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int v = list[i];
+__device__ inline void filter_one(const LpArgs& a, int v) {
     const int t = a.prop_t[v];
     if (t < 0) return;
     const int own = a.label[v];
@@ -936,10 +933,7 @@ __global__ void filter_departures(LpArgs a, const int* __restrict__ list, int co
     }
 }
 
-__global__ void admit_moves(LpArgs a, const int* __restrict__ list, int count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int v = list[i];
+__device__ inline void admit_one(const LpArgs& a, int v) {
     const int t = a.prop_t[v];
     if (t < 0 || !a.prop_ok[v]) return;
     const int w = a.vwgt[v];
@@ -950,6 +944,58 @@ __global__ void admit_moves(LpArgs a, const int* __restrict__ list, int count) {
     if (!ok) a.prop_ok[v] = 0;
 }
 
+// Applies v's proposal if admitted and clears its scratch; returns whether v moved.
+template <LpMode M>
+__device__ inline bool apply_one(const LpArgs& a, int v, int salt) {
+    const int t = a.prop_t[v];
+    if (t < 0) return false;
+    const int own = a.label[v];
+    a.inc[t] = 0;
+    a.top[t] = 0ull;
+    if (M == LpMode::Refine) {
+        a.out[own] = 0;
+        a.dtop[own] = 0ull;
+    }
+    a.prop_t[v] = -1;
+    if (!a.prop_ok[v]) return false;
+    const int w = a.vwgt[v];
+    atomicSub(&a.size[own], w);
+    atomicAdd(&a.size[t], w);
+    a.label[v] = t;
+    if (M == LpMode::Refine) {
+        const int beg = a.xadj[v], end = a.xadj[v + 1];
+        if (end - beg > kBigDegree) {
+            const int bi = atomicAdd(a.n_big, 1);
+            a.big[bi] = v;
+            if (a.hub_cnt) a.big_from[bi] = own;
+        } else {
+            atomicAdd(&a.trav[salt & 63], (unsigned long long)(end - beg));
+            for (int e = beg; e < end; ++e) {
+                const int u = a.adjncy[e];
+                a.next_active[u] = 1;
+                if (a.hub_cnt) {
+                    const int slot = a.hub_slot[u];
+                    if (slot >= 0) {
+                        atomicSub(&a.hub_cnt[slot + own], a.adjwgt[e]);
+                        atomicAdd(&a.hub_cnt[slot + t], a.adjwgt[e]);
+                    }
+                }
+            }
+        }
+    }
+    return true;
+}
+
+__global__ void filter_departures(LpArgs a, const int* __restrict__ list, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) filter_one(a, list[i]);
+}
+
+__global__ void admit_moves(LpArgs a, const int* __restrict__ list, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) admit_one(a, list[i]);
+}
+
 template <LpMode M>
 __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restrict__ list,
                                                    int count) {
@@ -957,46 +1003,67 @@ __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restri
     if (threadIdx.x == 0) block_moves = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int v = i < count ? list[i] : 0;
-    const int t = i < count ? a.prop_t[v] : -1;
-    if (t >= 0) {
-        const int own = a.label[v];
-        a.inc[t] = 0;
-        a.top[t] = 0ull;
-        if (M == LpMode::Refine) {
-            a.out[own] = 0;
-            a.dtop[own] = 0ull;
-        }
-        a.prop_t[v] = -1;
-        if (a.prop_ok[v]) {
-            const int w = a.vwgt[v];
-            atomicSub(&a.size[own], w);
-            atomicAdd(&a.size[t], w);
-            a.label[v] = t;
-            atomicAdd(&block_moves, 1);
-            if (M == LpMode::Refine) {
-                const int beg = a.xadj[v], end = a.xadj[v + 1];
-                if (end - beg > kBigDegree) {
-                    const int bi = atomicAdd(a.n_big, 1);
-                    a.big[bi] = v;
-                    if (a.hub_cnt) a.big_from[bi] = own;
-                } else {
-                    atomicAdd(&a.trav[(blockIdx.x + threadIdx.x) & 63], (unsigned long long)(end - beg));
-                    for (int e = beg; e < end; ++e) {
-                        const int u = a.adjncy[e];
-                        a.next_active[u] = 1;
-                        if (a.hub_cnt) {
-                            const int slot = a.hub_slot[u];
-                            if (slot >= 0) {
-                                atomicSub(&a.hub_cnt[slot + own], a.adjwgt[e]);
-                                atomicAdd(&a.hub_cnt[slot + t], a.adjwgt[e]);
-                            }
-                        }
-                    }
-                }
-            }
-        }
+    if (i < count && apply_one<M>(a, list[i], blockIdx.x + threadIdx.x)) atomicAdd(&block_moves, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
+}
+
+// The refinement's filter / admit / apply run over the sub-round's proposers only: the compacted
+// frontier of every degree list (lengths on the device) and the hub list (every proposer is on
+// one of them; nodes off them have no proposal, on which the three steps do nothing). Thread i
+// of the grid-stride loop takes element i of the lists' concatenation.
+struct SubLists {
+    const int* list[kNumBins];
+    const int* dcount[kNumBins];  // device-side length, or nullptr: hcount
+    int hcount[kNumBins];
+};
+__device__ inline int sub_lengths(const SubLists& L, int* cnt) {
+    int total = 0;
+#pragma unroll
+    for (int b = 0; b < kNumBins; ++b) {
+        cnt[b] = L.dcount[b] ? *L.dcount[b] : L.hcount[b];
+        total += cnt[b];
     }
+    return total;
+}
+__device__ inline int sub_node(const SubLists& L, const int* cnt, int i) {
+    // unrolled select (no dynamically indexed register array)
+    const int* lp = L.list[0];
+    bool found = false;
+#pragma unroll
+    for (int b = 0; b < kNumBins; ++b) {
+        const bool here = !found && i < cnt[b];
+        if (here) lp = L.list[b];
+        found |= here;
+        if (!found) i -= cnt[b];
+    }
+    return lp[i];
+}
+
+__global__ __launch_bounds__(256) void filter_departures_sub(LpArgs a, SubLists L) {
+    int cnt[kNumBins];
+    const int total = sub_lengths(L, cnt);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256)
+        filter_one(a, sub_node(L, cnt, i));
+}
+
+__global__ __launch_bounds__(256) void admit_moves_sub(LpArgs a, SubLists L) {
+    int cnt[kNumBins];
+    const int total = sub_lengths(L, cnt);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256)
+        admit_one(a, sub_node(L, cnt, i));
+}
+
+__global__ __launch_bounds__(256) void apply_moves_sub(LpArgs a, SubLists L) {
+    __shared__ int block_moves;
+    if (threadIdx.x == 0) block_moves = 0;
+    __syncthreads();
+    int cnt[kNumBins];
+    const int total = sub_lengths(L, cnt);
+    int mine = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256)
+        mine += apply_one<LpMode::Refine>(a, sub_node(L, cnt, i), blockIdx.x + threadIdx.x);
+    if (mine) atomicAdd(&block_moves, mine);
     __syncthreads();
     if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
 }
@@ -1774,12 +1841,22 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
             const int gm = grid_for(sub_total, 256);
             const int* sub_list = lists_.p + host_offsets[sr * kNumBins];  // bins are contiguous
             ProfScope ps(prof, base + "_admit_apply", 0.0, 0, s);
-            if (!cl) hipLaunchKernelGGL(filter_departures, dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
-            hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
             if (cl) {
+                hipLaunchKernelGGL(admit_moves, dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
                 hipLaunchKernelGGL((apply_moves<LpMode::Cluster>), dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
             } else {
-                hipLaunchKernelGGL((apply_moves<LpMode::Refine>), dim3(gm), dim3(256), 0, s, a, sub_list, sub_total);
+                SubLists L{};
+                for (int b = 0; b < kNumBins; ++b) {
+                    const int l = sr * kNumBins + b;
+                    const bool fr = b < kNumBins - 1;  // degree lists: the compacted frontier
+                    L.list[b] = (fr ? clists_.p : lists_.p) + host_offsets[l];
+                    L.dcount[b] = fr ? ccount_.p + l : nullptr;
+                    L.hcount[b] = host_counts[l];
+                }
+                const int gs = std::min(gm, 1024);
+                hipLaunchKernelGGL(filter_departures_sub, dim3(gs), dim3(256), 0, s, a, L);
+                hipLaunchKernelGGL(admit_moves_sub, dim3(gs), dim3(256), 0, s, a, L);
+                hipLaunchKernelGGL(apply_moves_sub, dim3(gs), dim3(256), 0, s, a, L);
                 hipLaunchKernelGGL(mark_big_neighbours, dim3(std::min(gm, 1024)), dim3(256), 0, s, a);
             }
         }
