@@ -232,14 +232,14 @@ __device__ __attribute__((always_inline)) inline void scan_wave_table(
 // A virtual block holds IT x (kBlock / G) nodes: every lane group serves IT nodes at once, whose
 // dependent loads (list, offsets, targets, labels, label weights) are issued together, so each
 // wave has that many chains in flight.
+// One virtual block (IT x kBlock / G nodes of the list) of the G-lane kernel; every thread of
+// the workgroup calls it (it synchronises the workgroup). `salt` picks the traversal counter.
 template <LpMode M, int G, int IT>
-__global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
-                                                        int count, const int* __restrict__ dcount) {
+__device__ __attribute__((always_inline)) inline void group_vb(const LpArgs& a,
+                                                               const int* __restrict__ list,
+                                                               int count, int vb, int salt) {
     constexpr int NB = kBlock / G;  // nodes per item and workgroup
-    if (dcount) count = *dcount;
-    const int nvb = (int)(((long long)count + IT * NB - 1) / (IT * NB));
     const int gl = threadIdx.x & (G - 1);
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
     int v[IT], own[IT], beg[IT], deg[IT], lab[IT], w[IT];
     bool valid[IT];
 #pragma unroll
@@ -366,25 +366,32 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
             if (valid[i] && gl == 0) mine += (unsigned long long)deg[i];
         if (mine) atomicAdd(&tsum, mine);
         __syncthreads();
-        if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[vb & 63], tsum);
+        if (threadIdx.x == 0 && tsum) atomicAdd(&a.trav[salt & 63], tsum);
     }
     __syncthreads();  // the LDS of this virtual block is read before the next one writes it
-    }
+}
+
+template <LpMode M, int G, int IT>
+__global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __restrict__ list,
+                                                        int count, const int* __restrict__ dcount) {
+    if (dcount) count = *dcount;
+    const int nvb = (int)(((long long)count + IT * (kBlock / G) - 1) / (IT * (kBlock / G)));
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) group_vb<M, G, IT>(a, list, count, vb, vb);
 }
 
 // ---------------------------------------------------------------- one wave per node
 // Labels are aggregated in LDS: a hash table (open addressing, LDS atomics) in general, or a
 // dense counter per label when the labels are the k blocks of the refinement and k is small.
-template <LpMode M, bool DENSE, int WCAP = kWaveCap>
-__global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
-                                                       int count, int num_labels,
-                                                       const int* __restrict__ dcount) {
+// One virtual block (kBlock / 64 nodes of the list, a wave each) of the wave kernel; every thread
+// of the workgroup calls it.
+template <LpMode M, bool DENSE, int WCAP>
+__device__ __attribute__((always_inline)) inline void wave_vb(const LpArgs& a,
+                                                              const int* __restrict__ list,
+                                                              int count, int num_labels, int vb,
+                                                              int salt) {
     static_assert(!DENSE || WCAP == kWaveCap, "the dense variant serves every wave degree");
     __shared__ int skey[kBlock / 64][DENSE ? 1 : WCAP];
     __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : WCAP];
-    if (dcount) count = *dcount;
-    const int nvb = (count + kBlock / 64 - 1) / (kBlock / 64);
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int gi = vb * (kBlock / 64) + wave;
@@ -463,9 +470,57 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
     reduce_cand<64>(c);
     if (valid && lane == 0) {
         decide<M>(a, v, own, nw, conn_own, c);
-        if (M == LpMode::Refine) atomicAdd(&a.trav[vb & 63], (unsigned long long)deg);
+        if (M == LpMode::Refine) atomicAdd(&a.trav[salt & 63], (unsigned long long)deg);
     }
     __syncthreads();  // the LDS of this virtual block is read before the next one clears it
+}
+
+template <LpMode M, bool DENSE, int WCAP = kWaveCap>
+__global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __restrict__ list,
+                                                       int count, int num_labels,
+                                                       const int* __restrict__ dcount) {
+    if (dcount) count = *dcount;
+    const int nvb = (count + kBlock / 64 - 1) / (kBlock / 64);
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        wave_vb<M, DENSE, WCAP>(a, list, count, num_labels, vb, vb);
+}
+
+// Refinement: the proposals of all degree lists of a sub-round in one launch. The lists are the
+// round's compacted frontier (lengths on the device); the workgroups stride over the virtual
+// blocks of the five lists in order (G = 8, 16, 32, 64 lanes per node, then a wave per node), so a
+// sub-round costs one launch instead of five. Same per-node code as the per-list kernels.
+struct FrontierLists {
+    const int* list[kNumBins - 1];
+    const int* dcount[kNumBins - 1];
+};
+template <int IT, bool DENSE>
+__global__ __launch_bounds__(kBlock) void propose_frontier(LpArgs a, FrontierLists F,
+                                                           int num_labels) {
+    constexpr int NPB[kNumBins - 1] = {IT * (kBlock / 8), IT * (kBlock / 16), IT * (kBlock / 32),
+                                       IT * (kBlock / 64), kBlock / 64};
+    int cnt[kNumBins - 1], nvb[kNumBins - 1], total = 0;
+#pragma unroll
+    for (int b = 0; b < kNumBins - 1; ++b) {
+        cnt[b] = *F.dcount[b];
+        nvb[b] = (cnt[b] + NPB[b] - 1) / NPB[b];
+        total += nvb[b];
+    }
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int vb = t, b = 0;  // uniform per workgroup
+#pragma unroll
+        for (int k = 0; k < kNumBins - 2; ++k)
+            if (b == k && vb >= nvb[k]) {
+                vb -= nvb[k];
+                b = k + 1;
+            }
+        switch (b) {
+            case 0: group_vb<LpMode::Refine, 8, IT>(a, F.list[0], cnt[0], vb, t); break;
+            case 1: group_vb<LpMode::Refine, 16, IT>(a, F.list[1], cnt[1], vb, t); break;
+            case 2: group_vb<LpMode::Refine, 32, IT>(a, F.list[2], cnt[2], vb, t); break;
+            case 3: group_vb<LpMode::Refine, 64, IT>(a, F.list[3], cnt[3], vb, t); break;
+            default:
+                wave_vb<LpMode::Refine, DENSE, kWaveCap>(a, F.list[4], cnt[4], num_labels, vb, t);
+        }
     }
 }
 
@@ -1335,6 +1390,20 @@ void launch_group(bool cl, int items, const LpArgs& a, const int* lst, int cnt, 
     }
 }
 
+void launch_frontier(int items, bool dense, const LpArgs& a, const FrontierLists& F, int nl,
+                     int listed, hipStream_t s) {
+    // virtual blocks of at least 4 nodes: a listed count bounds the grid
+    const dim3 grid(std::min(grid_for(listed, kBlock / 64), kFrontierGrid));
+    switch (items * 2 + (dense ? 1 : 0)) {
+        case 2: hipLaunchKernelGGL((propose_frontier<1, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 3: hipLaunchKernelGGL((propose_frontier<1, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 8: hipLaunchKernelGGL((propose_frontier<4, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 9: hipLaunchKernelGGL((propose_frontier<4, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 5: hipLaunchKernelGGL((propose_frontier<2, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        default: hipLaunchKernelGGL((propose_frontier<2, false>), grid, dim3(kBlock), 0, s, a, F, nl);
+    }
+}
+
 }  // namespace
 
 // ================================================================= host side
@@ -1736,11 +1805,25 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         for (int sr = 0; sr < S; ++sr) {
             a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
+            if (!cl) {
+                // refinement: the five degree lists' proposals in one launch over the frontier
+                FrontierLists F{};
+                int listed_sr = 0;
+                for (int b = 0; b < kNumBins - 1; ++b) {
+                    const int l = sr * kNumBins + b;
+                    F.list[b] = clists_.p + host_offsets[l];
+                    F.dcount[b] = ccount_.p + l;
+                    listed_sr += host_counts[l];
+                }
+                if (listed_sr > 0)
+                    launch_frontier(bins_.group_items, dense_wave_, a, F, nl, listed_sr, s);
+            }
             for (int b = 0; b < kNumBins; ++b) {
                 const int l = sr * kNumBins + b;
                 const int cnt = host_counts[l];
                 sub_total += cnt;
                 if (cnt == 0) continue;
+                if (!cl && b < kNumBins - 1) continue;  // proposed by the frontier launch above
                 // refinement: the round's frontier of the list (length on the device)
                 const bool fr = !cl && b < kNumBins - 1;
                 const int* lst = (fr ? clists_.p : lists_.p) + host_offsets[l];
