@@ -489,26 +489,60 @@ __global__ __launch_bounds__(kBlock) void propose_wave(LpArgs a, const int* __re
 // round's compacted frontier (lengths on the device); the workgroups stride over the virtual
 // blocks of the five lists in order (G = 8, 16, 32, 64 lanes per node, then a wave per node), so a
 // sub-round costs one launch instead of five. Same per-node code as the per-list kernels.
+// Refinement hubs with kept per-block counters (every hub of the level has them when their
+// counters fit): one wave per active hub decides from its k counters (as hub_dense_decide does
+// with keep = 1; no row is read).
+__device__ __attribute__((always_inline)) inline void hubcnt_vb(const LpArgs& a,
+                                                                const int* __restrict__ list,
+                                                                int count, int num_labels,
+                                                                int vb) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = vb * (kBlock / 64) + wave;
+    if (gi >= count) return;  // uniform per wave; no workgroup barrier below
+    const int v = list[gi];
+    if (a.active[v] == 0) return;
+    const int* cnt = a.hub_cnt + a.hub_slot[v];
+    const int own = a.label[v];
+    const int nw = a.vwgt[v];
+    int conn_own = 0;
+    Cand c{-1, 0u, 0x7fffffff, 0};
+    for (int b = lane; b < num_labels; b += 64) {
+        const int conn = cnt[b];
+        if (conn == 0) continue;
+        if (b == own) {
+            conn_own = conn;
+            continue;
+        }
+        const int sz = a.size[b];
+        if ((long long)sz + nw >= (long long)bound_of(a, b)) continue;
+        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)b);
+        if (better(conn, t, b, c.conn, c.tie, c.lab)) c = Cand{conn, t, b, sz};
+    }
+    conn_own = reduce_max<64>(conn_own);
+    reduce_cand<64>(c);
+    if (lane == 0) decide<LpMode::Refine>(a, v, own, nw, conn_own, c);
+}
+
 struct FrontierLists {
-    const int* list[kNumBins - 1];
-    const int* dcount[kNumBins - 1];
+    const int* list[kNumBins];
+    const int* dcount[kNumBins];  // dcount[kNumBins - 1] == nullptr: hubs decide elsewhere
 };
 template <int IT, bool DENSE>
 __global__ __launch_bounds__(kBlock) void propose_frontier(LpArgs a, FrontierLists F,
                                                            int num_labels) {
-    constexpr int NPB[kNumBins - 1] = {IT * (kBlock / 8), IT * (kBlock / 16), IT * (kBlock / 32),
-                                       IT * (kBlock / 64), kBlock / 64};
-    int cnt[kNumBins - 1], nvb[kNumBins - 1], total = 0;
+    constexpr int NPB[kNumBins] = {IT * (kBlock / 8), IT * (kBlock / 16), IT * (kBlock / 32),
+                                   IT * (kBlock / 64), kBlock / 64, kBlock / 64};
+    int cnt[kNumBins], nvb[kNumBins], total = 0;
 #pragma unroll
-    for (int b = 0; b < kNumBins - 1; ++b) {
-        cnt[b] = *F.dcount[b];
+    for (int b = 0; b < kNumBins; ++b) {
+        cnt[b] = F.dcount[b] ? *F.dcount[b] : 0;
         nvb[b] = (cnt[b] + NPB[b] - 1) / NPB[b];
         total += nvb[b];
     }
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int vb = t, b = 0;  // uniform per workgroup
 #pragma unroll
-        for (int k = 0; k < kNumBins - 2; ++k)
+        for (int k = 0; k < kNumBins - 1; ++k)
             if (b == k && vb >= nvb[k]) {
                 vb -= nvb[k];
                 b = k + 1;
@@ -518,8 +552,10 @@ __global__ __launch_bounds__(kBlock) void propose_frontier(LpArgs a, FrontierLis
             case 1: group_vb<LpMode::Refine, 16, IT>(a, F.list[1], cnt[1], vb, t); break;
             case 2: group_vb<LpMode::Refine, 32, IT>(a, F.list[2], cnt[2], vb, t); break;
             case 3: group_vb<LpMode::Refine, 64, IT>(a, F.list[3], cnt[3], vb, t); break;
-            default:
+            case 4:
                 wave_vb<LpMode::Refine, DENSE, kWaveCap>(a, F.list[4], cnt[4], num_labels, vb, t);
+                break;
+            default: hubcnt_vb(a, F.list[5], cnt[5], num_labels, vb);
         }
     }
 }
@@ -1259,7 +1295,8 @@ __global__ __launch_bounds__(256) void compact_active(const int* __restrict__ li
             if (off[mid] <= i) lo = mid; else hi = mid - 1;
         }
         v = lists[i];
-        if (lo % kNumBins != kNumBins - 1 && active[v]) {
+        if (active[v]) {  // every list, the hub list included (its frontier feeds the
+                          // counter-based hub decisions)
             l = lo;
             r = atomicAdd(&local[l], 1);
         }
@@ -1789,6 +1826,8 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         }
     }
     const int nl = num_labels_;
+    // every refinement hub decides from kept counters: those decisions join the frontier launch
+    const bool fused_hubs = incremental && dense_all_;
     const int nseg = S * kNumBins;
     const int listed = host_offsets[nseg - 1] + host_counts[nseg - 1];
     for (int r = 0; r < rounds; ++r) {
@@ -1806,14 +1845,16 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
             a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
             if (!cl) {
-                // refinement: the five degree lists' proposals in one launch over the frontier
+                // refinement: the five degree lists' proposals in one launch over the frontier,
+                // and the hubs' decisions from their kept counters when every hub has them
                 FrontierLists F{};
                 int listed_sr = 0;
-                for (int b = 0; b < kNumBins - 1; ++b) {
+                for (int b = 0; b < kNumBins; ++b) {
                     const int l = sr * kNumBins + b;
+                    const bool take = b < kNumBins - 1 || fused_hubs;
                     F.list[b] = clists_.p + host_offsets[l];
-                    F.dcount[b] = ccount_.p + l;
-                    listed_sr += host_counts[l];
+                    F.dcount[b] = take ? ccount_.p + l : nullptr;
+                    listed_sr += take ? host_counts[l] : 0;
                 }
                 if (listed_sr > 0)
                     launch_frontier(bins_.group_items, dense_wave_, a, F, nl, listed_sr, s);
@@ -1823,7 +1864,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 const int cnt = host_counts[l];
                 sub_total += cnt;
                 if (cnt == 0) continue;
-                if (!cl && b < kNumBins - 1) continue;  // proposed by the frontier launch above
+                if (!cl && (b < kNumBins - 1 || fused_hubs)) continue;  // in the frontier launch
                 // refinement: the round's frontier of the list (length on the device)
                 const bool fr = !cl && b < kNumBins - 1;
                 const int* lst = (fr ? clists_.p : lists_.p) + host_offsets[l];
