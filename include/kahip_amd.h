@@ -110,6 +110,12 @@ void kahip_graph_free(kahip_graph_t g);
 
 /* kaffpa's pipeline on a device graph. part: host array of n ints (may be NULL). */
 int kahip_partition(kahip_graph_t g, const kahip_params* params, int* part, kahip_stats* stats);
+
+/* kaffpa, kaffpa_balance(_NE), kahip_partition and kahip_batch_partition keep their device
+ * workspaces (and stream) per device between calls, so that repeated calls do not allocate them
+ * again; this frees those of the engines no call is using. No counterpart in the reference
+ * (its kaffpa keeps no state between calls). */
+int kahip_release_workspaces(void);
 /* JSON object {name: {launches, ms, bytes, edges}} of the last profiled call on this thread. */
 int kahip_profile_json(char* buf, int len);
 

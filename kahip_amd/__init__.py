@@ -97,7 +97,8 @@ class Stats(C.Structure):
 EXPORTS = [
     "kaffpa", "kaffpa_balance", "kaffpa_balance_NE", "kahip_default_params", "kahip_last_error",
     "kaffpa_gpu_ex", "kahip_graph_from_host", "kahip_graph_rmat", "kahip_graph_info",
-    "kahip_graph_to_host", "kahip_graph_free", "kahip_graph_rgg", "kahip_partition", "kahip_profile_json",
+    "kahip_graph_to_host", "kahip_graph_free", "kahip_graph_rgg", "kahip_partition",
+    "kahip_release_workspaces", "kahip_profile_json",
     "kahip_lp_cluster", "kahip_remap", "kahip_contract", "kahip_lp_refine", "kahip_edge_cut",
     "kahip_block_weights", "kahip_project", "kahip_initial_partition", "kahip_edge_features",
     "kahip_graph_edge_features", "kahip_mlp_forward", "kahip_edge_scores", "kahip_mlp_gradient",
@@ -146,6 +147,7 @@ def lib():
     L.kahip_graph_to_host.argtypes = [G, ip, ip, ip, ip]
     L.kahip_graph_free.argtypes = [G]
     L.kahip_graph_free.restype = None
+    L.kahip_release_workspaces.argtypes = []
     L.kahip_partition.argtypes = [G, C.POINTER(Params), ip, C.POINTER(Stats)]
     L.kahip_profile_json.argtypes = [C.c_char_p, C.c_int]
     L.kahip_lp_cluster.argtypes = [C.c_int, _i32p, _i32p, ip, ip, C.c_int, C.c_int, C.c_int,
@@ -340,6 +342,12 @@ class Graph:
             self.free()
         except Exception:
             pass
+
+
+def release_workspaces():
+    """Frees the device workspaces that the partition entry points keep per device between
+    calls (those of engines no call is using)."""
+    _check(lib().kahip_release_workspaces())
 
 
 def profile():

@@ -1,7 +1,8 @@
 // extern "C" entry points of libkahip.so (declared in include/kaHIP_interface.h and
-// include/kahip_amd.h). Host arrays are copied to the GPU, the work runs on a per-call stream,
-// results are copied back; no state survives a call except the thread-local last error and
-// last profile summary.
+// include/kahip_amd.h). Host arrays are copied to the GPU, the work runs on a stream, results
+// are copied back. State that survives a call: the thread-local last error and last profile
+// summary, and the partition entry points' cached engine per device (its stream and device
+// workspaces, reused by the next call on that device; kahip_release_workspaces frees them).
 
 #include <algorithm>
 #include <climits>
@@ -9,6 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <queue>
 #include <sstream>
 #include <string>
@@ -50,6 +55,62 @@ int guarded(F&& f) {
     }
     return 1;
 }
+
+// The partition entry points (kaffpa, kahip_partition, kahip_batch_partition) run on a cached
+// engine of the current device, so that a repeated call reuses the device workspaces sized by
+// the previous ones instead of allocating (and freeing) them inside the call; every workspace is
+// re-initialised by the kernels that use it, exactly as between two levels of one call. A call
+// that finds the device's engine held by another thread runs on an engine of its own, and a call
+// that fails drops the cached engine. The cache is never destroyed at exit (its memory goes with
+// the process); kahip_release_workspaces frees the idle engines.
+struct CachedEngine {
+    std::mutex mu;
+    std::unique_ptr<Engine> eng;
+};
+std::mutex g_cache_mu;
+std::map<int, CachedEngine*>& engine_cache() {
+    static auto* cache = new std::map<int, CachedEngine*>();  // left to the process exit
+    return *cache;
+}
+
+class EngineLease {
+public:
+    EngineLease() : unwinding_(std::uncaught_exceptions()) {
+        int dev = 0;
+        KH_CHECK(hipGetDevice(&dev));
+        CachedEngine* c;
+        {
+            std::lock_guard<std::mutex> g(g_cache_mu);
+            CachedEngine*& slot = engine_cache()[dev];
+            if (!slot) slot = new CachedEngine();
+            c = slot;
+        }
+        std::unique_lock<std::mutex> lk(c->mu, std::try_to_lock);
+        if (lk.owns_lock()) {
+            if (!c->eng) c->eng.reset(new Engine());
+            cached_ = c;
+            e_ = c->eng.get();
+            lk_ = std::move(lk);
+        } else {
+            own_.reset(new Engine());
+            e_ = own_.get();
+        }
+    }
+    ~EngineLease() {
+        if (cached_ && std::uncaught_exceptions() > unwinding_) cached_->eng.reset();
+    }
+    EngineLease(const EngineLease&) = delete;
+    EngineLease& operator=(const EngineLease&) = delete;
+    Engine& operator*() { return *e_; }
+    Engine* operator->() { return e_; }
+
+private:
+    int unwinding_;
+    CachedEngine* cached_ = nullptr;
+    Engine* e_ = nullptr;
+    std::unique_ptr<Engine> own_;
+    std::unique_lock<std::mutex> lk_;
+};
 
 PartitionParams to_params(const kahip_params* q) {
     kahip_params d;
@@ -159,12 +220,12 @@ int run_kaffpa(int n, const int* vwgt, const int* xadj, const int* adjcwgt, cons
     return guarded([&] {
         check_csr(n, xadj, adjncy);
         PartitionParams p = to_params(params);
-        Engine eng;
+        EngineLease eng;
         OwnedGraph g;
-        upload_graph(n, xadj, adjncy, vwgt, adjcwgt, g, eng.stream());
+        upload_graph(n, xadj, adjncy, vwgt, adjcwgt, g, eng->stream());
         PartitionStats st;
-        eng.partition(g.g, p, part, &st);
-        if (p.profile) g_last_profile = profile_json(eng.prof);
+        eng->partition(g.g, p, part, &st);
+        if (p.profile) g_last_profile = profile_json(eng->prof);
         if (edgecut) *edgecut = (int)st.cut;
         fill_stats(st, stats);
     });
@@ -373,19 +434,29 @@ int kahip_graph_to_host(kahip_graph_t g, int* xadj, int* adjncy, int* vwgt, int*
 
 void kahip_graph_free(kahip_graph_t g) { delete g; }
 
+int kahip_release_workspaces(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        for (auto& kv : engine_cache()) {
+            std::unique_lock<std::mutex> lk(kv.second->mu, std::try_to_lock);
+            if (lk.owns_lock()) kv.second->eng.reset();  // an engine in use stays
+        }
+    });
+}
+
 int kahip_partition(kahip_graph_t g, const kahip_params* params, int* part, kahip_stats* stats) {
     return guarded([&] {
         KH_REQUIRE(g, "null graph");
         PartitionParams p = to_params(params);
-        Engine eng;
+        EngineLease eng;
         std::vector<int> tmp;
         if (!part) {
             tmp.resize(std::max(g->og.g.n, 1));
             part = tmp.data();
         }
         PartitionStats st;
-        eng.partition(g->og.g, p, part, &st);
-        if (p.profile) g_last_profile = profile_json(eng.prof);
+        eng->partition(g->og.g, p, part, &st);
+        if (p.profile) g_last_profile = profile_json(eng->prof);
         fill_stats(st, stats);
     });
 }
@@ -833,7 +904,7 @@ int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_g
     return guarded([&] {
         KH_REQUIRE(b, "null batch");
         PartitionParams p = to_params(params);
-        Engine eng;
+        EngineLease eng;
         std::vector<int> tmp;
         if (!part) {
             tmp.resize(std::max(b->og.g.n, 1));
@@ -841,7 +912,7 @@ int kahip_batch_partition(kahip_batch_t b, const kahip_params* params, int use_g
         }
         BatchStats st;
         const int G = (int)b->node_off.size() - 1;
-        partition_batch(eng, b->og.g, G, b->node_off.data(), p, part, &st, use_graph != 0);
+        partition_batch(*eng, b->og.g, G, b->node_off.data(), p, part, &st, use_graph != 0);
         for (int g = 0; g < G; ++g) {
             if (cuts) cuts[g] = st.cut[g];
             if (max_block_weights) max_block_weights[g] = st.max_block_weight[g];

@@ -172,22 +172,27 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     // ------------------------------------------------------------------ coarsening
     // Skewed graphs are renumbered by degree first (part of the coarsening time); the pipeline
     // runs on the renumbered graph and the partition is mapped back at the end.
-    OwnedGraph reordered;
-    DevBuf<int> inv;
+    OwnedGraph& reordered = buf_.reordered;
+    DevBuf<int>& inv = buf_.inv;
     const bool reorder = reorder_wanted(input, p.reorder);
     EventPair ev_coarsen;
     KH_CHECK(hipEventRecord(ev_coarsen.a, s));
-    if (reorder) reorder_by_degree(input, reordered, inv, s, prof);
+    if (reorder) reorder_by_degree(input, reordered, inv, buf_.rtmp, s, prof);
     const DGraph& g0 = reorder ? reordered.g : input;
     S.reordered = reorder;
-    std::vector<std::unique_ptr<OwnedGraph>> coarse;  // coarse[l] = level l+1
-    std::vector<DevBuf<int>> maps;                     // maps[l]: level l -> level l+1
+    auto& coarse = buf_.coarse;  // coarse[l] = level l+1 (entries past the last level: unused)
+    auto& maps = buf_.maps;      // maps[l]: level l -> level l+1
     auto level_graph = [&](int l) -> const DGraph& { return l == 0 ? g0 : coarse[l - 1]->g; };
-    DevBuf<int> label, size, score, elabel;
+    DevBuf<int>& label = buf_.label;
+    DevBuf<int>& size = buf_.size;
+    DevBuf<int>& score = buf_.score;
+    DevBuf<int>& elabel = buf_.elabel;
     // label := the smallest node id among the nodes sharing (label, other)
-    DevBuf<unsigned long long> okeys, okeys2;
-    DevBuf<int> oids, oids2;
-    DevBuf<char> otmp;
+    DevBuf<unsigned long long>& okeys = buf_.okeys;
+    DevBuf<unsigned long long>& okeys2 = buf_.okeys2;
+    DevBuf<int>& oids = buf_.oids;
+    DevBuf<int>& oids2 = buf_.oids2;
+    DevBuf<char>& otmp = buf_.otmp;
     auto overlay = [&](int n, int* lab, const int* other, hipStream_t st) {
         okeys.reserve(n);
         okeys2.reserve(n);
@@ -250,13 +255,13 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             S.edges_cluster += edges_round * p.cluster_rounds;
             overlay(fine.n, label.p, elabel.p, s);
         }
-        maps.emplace_back();
-        maps.back().reserve(fine.n);
-        const int nc = cw.remap(fine, label.p, maps.back().p, s, prof);
-        coarse.emplace_back(new OwnedGraph());
-        cw.contract(fine, maps.back().p, nc, *coarse.back(), s, prof);
+        if ((int)maps.size() <= level) maps.emplace_back();
+        if ((int)coarse.size() <= level) coarse.emplace_back(new OwnedGraph());
+        maps[level].reserve(fine.n);
+        const int nc = cw.remap(fine, label.p, maps[level].p, s, prof);
+        cw.contract(fine, maps[level].p, nc, *coarse[level], s, prof);
         S.edges_contract += fine.m2;
-        DGraph& cg = coarse.back()->g;
+        DGraph& cg = coarse[level]->g;
         cg.total_vwgt = fine.total_vwgt;
         ++level;
         S.level_n[level] = cg.n;
@@ -284,7 +289,8 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     ip.threads = p.ip_threads;
     ip.bisection_tries = p.ip_tries;
     initial_partition(hg, ip, hpart);
-    DevBuf<int> part_c, part_f;
+    DevBuf<int>& part_c = buf_.part_c;
+    DevBuf<int>& part_f = buf_.part_f;
     part_c.reserve(std::max(cg.n, 1));
     if (cg.n > 0)
         KH_CHECK(hipMemcpyAsync(part_c.p, hpart.data(), sizeof(int) * cg.n, hipMemcpyHostToDevice, s));
@@ -295,7 +301,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     // Every level: projection, LP refinement on the GPU (timed by an event pair per level) and,
     // for the FM-based presets, the CPU pairwise FM of the level's partition, as the reference
     // refines every level after projection (uncoarsening.cpp:58-95).
-    DevBuf<int> W;
+    DevBuf<int>& W = buf_.W;
     W.reserve(p.k);
     HostGraph hl;
     std::vector<int> hp;

@@ -13,6 +13,7 @@
 #include "lp.hpp"
 #include "metrics.hpp"
 #include "rating.hpp"
+#include "reorder.hpp"
 
 namespace kahip {
 
@@ -117,6 +118,19 @@ public:
 
 private:
     hipStream_t s_ = nullptr;
+    // Device buffers of partition() (grown, never shrunk: a later call on a graph no larger
+    // allocates nothing); every one is written before it is read within a call.
+    struct Buffers {
+        OwnedGraph reordered;
+        DevBuf<int> inv;
+        ReorderScratch rtmp;
+        std::vector<std::unique_ptr<OwnedGraph>> coarse;  // coarse[l] = level l+1
+        std::vector<DevBuf<int>> maps;                     // maps[l]: level l -> level l+1
+        DevBuf<int> label, size, score, elabel, part_c, part_f, W;
+        DevBuf<unsigned long long> okeys, okeys2;
+        DevBuf<int> oids, oids2;
+        DevBuf<char> otmp;
+    } buf_;
 };
 
 // Uploads a host CSR (vwgt / adjwgt may be null: unit weights) into an owned device graph.

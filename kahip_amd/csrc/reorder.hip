@@ -109,13 +109,16 @@ bool reorder_wanted(const DGraph& g, int mode) {
     return (long long)g.max_degree * (long long)g.n > 32LL * g.m2;
 }
 
-void reorder_by_degree(const DGraph& g, OwnedGraph& out, DevBuf<int>& inv, hipStream_t s,
-                       Profiler* prof) {
+void reorder_by_degree(const DGraph& g, OwnedGraph& out, DevBuf<int>& inv, ReorderScratch& t,
+                       hipStream_t s, Profiler* prof) {
     ProfScope ps(prof, "reorder", 0.0, g.m2, s);
     const int n = g.n;
-    DevBuf<unsigned> keys, skeys;
-    DevBuf<int> ids, order, ndeg;
-    DevBuf<char> tmp;
+    DevBuf<unsigned>& keys = t.keys;
+    DevBuf<unsigned>& skeys = t.skeys;
+    DevBuf<int>& ids = t.ids;
+    DevBuf<int>& order = t.order;
+    DevBuf<int>& ndeg = t.ndeg;
+    DevBuf<char>& tmp = t.tmp;
     keys.reserve(n);
     skeys.reserve(n);
     ids.reserve(n);
@@ -147,7 +150,6 @@ void reorder_by_degree(const DGraph& g, OwnedGraph& out, DevBuf<int>& inv, hipSt
                            out.g.adjwgt);
     out.g.total_vwgt = g.total_vwgt;
     out.g.max_degree = g.max_degree;
-    KH_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
     KH_CHECK(hipGetLastError());
 }
 

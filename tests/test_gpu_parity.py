@@ -330,6 +330,32 @@ def test_pipeline_deterministic(G):
     assert np.array_equal(p1, p2) and s1["cut"] == s2["cut"]
 
 
+def test_cached_workspaces_are_reused_across_graphs(G):
+    """The partition entry points keep each device's engine (stream and device workspaces)
+    between calls: a partition equals the mirror's before and after larger and smaller graphs ran
+    on the same workspaces, after a failed call (which drops the cached engine) and after
+    release_workspaces."""
+    x, a, vw, ew = G["rmat12"]
+    want, _ = mirror_partition(x, a, vw, ew, 16, 0.03, 5)
+
+    def check():
+        got, st = K.partition(x, a, vw, ew, k=16, imbalance=0.03, seed=5)
+        assert np.array_equal(got, want)
+        assert st["cut"] == O.edge_cut(x, a, ew, want)
+
+    check()
+    for name, k in (("rgg15", 64), ("manual", 2), ("delaunay15", 8)):
+        gx, ga, gvw, gew = G[name]
+        K.partition(gx, ga, gvw, gew, k=k, imbalance=0.03, seed=1)
+        check()
+    with pytest.raises(K.KahipError):
+        K.partition(x, a, vw, ew, k=0)
+    check()
+    K.release_workspaces()
+    check()
+    check()
+
+
 GRAPH_OFF = ("hipGraph-captured LP rounds are disabled: that path hit a host segmentation fault "
              "under rocprofv3 whose cause was not found")
 
