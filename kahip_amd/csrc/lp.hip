@@ -1035,11 +1035,13 @@ __device__ inline void admit_one(const LpArgs& a, int v) {
     if (!ok) a.prop_ok[v] = 0;
 }
 
-// Applies v's proposal if admitted and clears its scratch; returns whether v moved.
+// Applies v's proposal if admitted and clears its scratch. Returns 0 if v did not move, 1 if it
+// moved, 2 if it moved and (refinement) its row of more than kBigDegree entries is left to the
+// caller's workgroup (old label in *big_from).
 template <LpMode M>
-__device__ inline bool apply_one(const LpArgs& a, int v, int salt) {
+__device__ inline int apply_one(const LpArgs& a, int v, int salt, int* big_from) {
     const int t = a.prop_t[v];
-    if (t < 0) return false;
+    if (t < 0) return 0;
     const int own = a.label[v];
     a.inc[t] = 0;
     a.top[t] = 0ull;
@@ -1048,7 +1050,7 @@ __device__ inline bool apply_one(const LpArgs& a, int v, int salt) {
         a.dtop[own] = 0ull;
     }
     a.prop_t[v] = -1;
-    if (!a.prop_ok[v]) return false;
+    if (!a.prop_ok[v]) return 0;
     const int w = a.vwgt[v];
     atomicSub(&a.size[own], w);
     atomicAdd(&a.size[t], w);
@@ -1056,9 +1058,8 @@ __device__ inline bool apply_one(const LpArgs& a, int v, int salt) {
     if (M == LpMode::Refine) {
         const int beg = a.xadj[v], end = a.xadj[v + 1];
         if (end - beg > kBigDegree) {
-            const int bi = atomicAdd(a.n_big, 1);
-            a.big[bi] = v;
-            if (a.hub_cnt) a.big_from[bi] = own;
+            *big_from = own;
+            return 2;
         } else {
             atomicAdd(&a.trav[salt & 63], (unsigned long long)(end - beg));
             for (int e = beg; e < end; ++e) {
@@ -1074,7 +1075,24 @@ __device__ inline bool apply_one(const LpArgs& a, int v, int salt) {
             }
         }
     }
-    return true;
+    return 1;
+}
+
+// Frontier marking (and hub counter updates) of moved node v's row by the whole workgroup.
+__device__ inline void mark_row(const LpArgs& a, int v, int from, int to) {
+    const int beg = a.xadj[v], end = a.xadj[v + 1];
+    if (threadIdx.x == 0) atomicAdd(&a.trav[(v + blockIdx.x) & 63], (unsigned long long)(end - beg));
+    for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
+        const int u = a.adjncy[e];
+        a.next_active[u] = 1;
+        if (a.hub_cnt) {
+            const int slot = a.hub_slot[u];
+            if (slot >= 0) {
+                atomicSub(&a.hub_cnt[slot + from], a.adjwgt[e]);
+                atomicAdd(&a.hub_cnt[slot + to], a.adjwgt[e]);
+            }
+        }
+    }
 }
 
 __global__ void filter_departures(LpArgs a, const int* __restrict__ list, int count) {
@@ -1094,7 +1112,9 @@ __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restri
     if (threadIdx.x == 0) block_moves = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count && apply_one<M>(a, list[i], blockIdx.x + threadIdx.x)) atomicAdd(&block_moves, 1);
+    int from;
+    if (i < count && apply_one<M>(a, list[i], blockIdx.x + threadIdx.x, &from))
+        atomicAdd(&block_moves, 1);
     __syncthreads();
     if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
 }
@@ -1145,42 +1165,40 @@ __global__ __launch_bounds__(256) void admit_moves_sub(LpArgs a, SubLists L) {
         admit_one(a, sub_node(L, cnt, i));
 }
 
+// Moved nodes of more than kBigDegree entries have their rows marked by the whole workgroup
+// right after each 256-node step (at most one such node per thread and step).
 __global__ __launch_bounds__(256) void apply_moves_sub(LpArgs a, SubLists L) {
     __shared__ int block_moves;
+    __shared__ int nbig;
+    __shared__ int bigv[256], bigfrom[256], bigto[256];
     if (threadIdx.x == 0) block_moves = 0;
-    __syncthreads();
     int cnt[kNumBins];
     const int total = sub_lengths(L, cnt);
     int mine = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256)
-        mine += apply_one<LpMode::Refine>(a, sub_node(L, cnt, i), blockIdx.x + threadIdx.x);
+    for (int base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {  // uniform
+        if (threadIdx.x == 0) nbig = 0;
+        __syncthreads();
+        const int i = base + threadIdx.x;
+        if (i < total) {
+            const int v = sub_node(L, cnt, i);
+            int from;
+            const int r = apply_one<LpMode::Refine>(a, v, blockIdx.x + threadIdx.x, &from);
+            mine += r != 0;
+            if (r == 2) {
+                const int k = atomicAdd(&nbig, 1);
+                bigv[k] = v;
+                bigfrom[k] = from;
+                bigto[k] = a.label[v];
+            }
+        }
+        __syncthreads();
+        const int nb = nbig;
+        for (int k = 0; k < nb; ++k) mark_row(a, bigv[k], bigfrom[k], bigto[k]);
+        __syncthreads();  // the step's big movers are done before nbig is reset
+    }
     if (mine) atomicAdd(&block_moves, mine);
     __syncthreads();
     if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
-}
-
-// Frontier marking for moved high-degree nodes: grid-stride over the big movers, one workgroup
-// per node at a time.
-__global__ __launch_bounds__(256) void mark_big_neighbours(LpArgs a) {
-    const int nb = *a.n_big;
-    for (int i = blockIdx.x; i < nb; i += gridDim.x) {
-        const int v = a.big[i];
-        const int to = a.label[v];
-        const int from = a.hub_cnt ? a.big_from[i] : 0;
-        if (threadIdx.x == 0)
-            atomicAdd(&a.trav[i & 63], (unsigned long long)(a.xadj[v + 1] - a.xadj[v]));
-        for (int e = a.xadj[v] + threadIdx.x; e < a.xadj[v + 1]; e += 256) {
-            const int u = a.adjncy[e];
-            a.next_active[u] = 1;
-            if (a.hub_cnt) {
-                const int slot = a.hub_slot[u];
-                if (slot >= 0) {
-                    atomicSub(&a.hub_cnt[slot + from], a.adjwgt[e]);
-                    atomicAdd(&a.hub_cnt[slot + to], a.adjwgt[e]);
-                }
-            }
-        }
-    }
 }
 
 __global__ void set_hub_slots(const int* __restrict__ list, const int* __restrict__ multi,
@@ -1492,8 +1510,6 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
     dtop_.reserve(num_labels);
     active_.reserve(g.n);
     next_active_.reserve(g.n);
-    big_.reserve(g.n);
-    n_big_.reserve(kMaxSub * kMaxRounds);
     n_big_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     n_mid_tasks_.reserve((long long)kMaxSub * kMaxRounds);
     trav_.reserve(64);
@@ -1589,7 +1605,6 @@ void LpWorkspace::prepare(LpMode mode, const DGraph& g, int num_labels, int S, u
         dense_cnt_.reserve(std::max<long long>(dtotal, 1));
         fill_async(dense_cnt_.p, std::max<long long>(dtotal, 1), 0, s);
         hub_slot_.reserve(g.n);
-        big_from_.reserve(g.n);
         fill_async(hub_slot_.p, g.n, -1, s);
         KH_CHECK(hipMemcpyAsync(chunk_prefix_.p, prefix.data(), sizeof(int) * prefix.size(),
                                 hipMemcpyHostToDevice, s));
@@ -1776,7 +1791,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     const bool cl = mode == LpMode::Cluster;
     KH_REQUIRE(cl || !(dense_wave_ || dense_hub_) || num_labels_ > 0, "prepare() not called");
     // no allocation here: prepare() sized everything
-    fill_async(n_big_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_big_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(n_mid_tasks_.p, (long long)kMaxSub * kMaxRounds, 0, s);
     fill_async(moves_.p, kMaxRounds, 0, s);
@@ -1803,7 +1817,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     a.prop_gain = prop_gain_.p;
     a.prop_ok = prop_ok_.p;
     a.trav = trav_.p;
-    a.big = big_.p;
     a.err = err_.p;
     // Refinement hubs of more than one chunk keep per-block connection counters across the
     // rounds: one scan builds them, every applied move updates its hub neighbours' counters, and
@@ -1811,7 +1824,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
     const bool incremental = !cl && dense_hub_ && dense_total_ > 0;
     a.hub_slot = incremental ? hub_slot_.p : nullptr;
     a.hub_cnt = incremental ? dense_cnt_.p : nullptr;
-    a.big_from = incremental ? big_from_.p : nullptr;
     uint8_t* act = active_.p;
     uint8_t* nxt = next_active_.p;
     const std::string base = std::string(phase ? phase : (cl ? "lp_cluster" : "lp_refine"));
@@ -1842,7 +1854,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                                lists_.p, offsets_.p, nseg, listed, act, ccount_.p, clists_.p);
         }
         for (int sr = 0; sr < S; ++sr) {
-            a.n_big = n_big_.p + (r * S + sr);
             int sub_total = 0;
             if (!cl) {
                 // refinement: the five degree lists' proposals in one launch over the frontier,
@@ -1981,7 +1992,6 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 hipLaunchKernelGGL(filter_departures_sub, dim3(gs), dim3(256), 0, s, a, L);
                 hipLaunchKernelGGL(admit_moves_sub, dim3(gs), dim3(256), 0, s, a, L);
                 hipLaunchKernelGGL(apply_moves_sub, dim3(gs), dim3(256), 0, s, a, L);
-                hipLaunchKernelGGL(mark_big_neighbours, dim3(std::min(gm, 1024)), dim3(256), 0, s, a);
             }
         }
         if (!cl) {

@@ -30,15 +30,12 @@ struct LpArgs {
     int* prop_gain;
     uint8_t* prop_ok;
     int* moves;
-    int* big;                  // refinement: moved nodes of degree > kBigDegree
-    int* n_big;
     unsigned long long* trav;  // refinement: traversed directed edges, 64 spread counters
     // Refinement with incremental hub counters (nullptr otherwise): hub_slot[v] is the offset of
     // node v's per-block connection counters in hub_cnt, or -1; moves update the counters of
-    // their hub neighbours; big_from[i] is the old block of big mover big[i].
+    // their hub neighbours.
     const int* hub_slot;
     int* hub_cnt;
-    int* big_from;
     int* err;  // device error flag (kErrProbe), cleared by run(), read by check()
 };
 
@@ -98,7 +95,7 @@ private:
     int ct_sr_base_[kMaxSub] = {};  // bucket-major count block of each sub-round
     int ct_sr_size_[kMaxSub] = {};
     size_t scan_tmp_bytes_ = 0;
-    DevBuf<int> hub_deg_, chunk_prefix_, multi_, big_, n_big_;
+    DevBuf<int> hub_deg_, chunk_prefix_, multi_;
     DevBuf<long long> tab_off_, tab_cap_;
     DevBuf<int2> partials_;
     DevBuf<int> ct_base_, cnt_t_, pos_t_;
@@ -121,7 +118,7 @@ private:
     DevBuf<uint8_t> active_, next_active_;
     DevBuf<int> hub_key_, hub_val_;
     DevBuf<int> dense_off_, dense_cnt_;  // chunked dense refinement hubs: per-hub counters
-    DevBuf<int> hub_slot_, big_from_;
+    DevBuf<int> hub_slot_;
     DevBuf<int> err_;
     long long dense_total_ = 0;
     bool dense_all_ = false;
