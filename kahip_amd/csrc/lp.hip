@@ -1119,10 +1119,10 @@ __global__ __launch_bounds__(256) void apply_moves(LpArgs a, const int* __restri
     if (threadIdx.x == 0 && block_moves) atomicAdd(a.moves, block_moves);
 }
 
-// The refinement's filter / admit / apply run over the sub-round's proposers only: the compacted
-// frontier of every degree list (lengths on the device) and the hub list (every proposer is on
-// one of them; nodes off them have no proposal, on which the three steps do nothing). Thread i
-// of the grid-stride loop takes element i of the lists' concatenation.
+// The refinement's filter / admit / apply run over the sub-round's compacted frontier of every
+// list, the hub list included (lengths on the device; every proposer is active, and nodes off
+// the frontier have no proposal, on which the three steps do nothing). Thread i of the
+// grid-stride loop takes element i of the lists' concatenation.
 struct SubLists {
     const int* list[kNumBins];
     const int* dcount[kNumBins];  // device-side length, or nullptr: hcount
@@ -1291,6 +1291,7 @@ __global__ void bin_scatter(int n, const int* __restrict__ xadj, uint32_t seed, 
 // Frontier of a refinement round: the active nodes of every non-hub list, compacted per list
 // into clists (same offsets as lists); ccount[l] receives list l's length. The order inside a
 // list is free: the synchronous schedule does not depend on it.
+constexpr int kCompactItems = 8;  // list entries per thread of compact_active
 __global__ __launch_bounds__(256) void compact_active(const int* __restrict__ lists,
                                                       const int* __restrict__ offsets, int nseg,
                                                       int total, const uint8_t* __restrict__ active,
@@ -1304,26 +1305,37 @@ __global__ __launch_bounds__(256) void compact_active(const int* __restrict__ li
         local[i] = 0;
     }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int l = -1, r = 0, v = 0;
-    if (i < total) {
-        int lo = 0, hi = nseg - 1;  // last list with off <= i
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (off[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        v = lists[i];
-        if (active[v]) {  // every list, the hub list included (its frontier feeds the
-                          // counter-based hub decisions)
-            l = lo;
-            r = atomicAdd(&local[l], 1);
+    // kCompactItems entries per thread, 256 apart (coalesced), their flag gathers in flight
+    // together; one global count update per list and workgroup
+    const int i0 = blockIdx.x * 256 * kCompactItems + threadIdx.x;
+    int v[kCompactItems], l[kCompactItems], r[kCompactItems];
+#pragma unroll
+    for (int j = 0; j < kCompactItems; ++j) {
+        const int i = i0 + 256 * j;
+        v[j] = i < total ? lists[i] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kCompactItems; ++j) {
+        const int i = i0 + 256 * j;
+        l[j] = -1;
+        if (v[j] >= 0 && active[v[j]]) {  // every list, the hub list included (its frontier
+                                          // feeds the counter-based hub decisions)
+            int lo = 0, hi = nseg - 1;  // last list with off <= i
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= i) lo = mid; else hi = mid - 1;
+            }
+            l[j] = lo;
+            r[j] = atomicAdd(&local[lo], 1);
         }
     }
     __syncthreads();
     for (int j = threadIdx.x; j < nseg; j += blockDim.x)
         base[j] = local[j] ? atomicAdd(&ccount[j], local[j]) : 0;
     __syncthreads();
-    if (l >= 0) clists[off[l] + base[l] + r] = v;
+#pragma unroll
+    for (int j = 0; j < kCompactItems; ++j)
+        if (l[j] >= 0) clists[off[l[j]] + base[l[j]] + r[j]] = v[j];
 }
 
 // Block descriptors of the hub kernels, built once per level: chunk c of the sub-round's hub
@@ -1850,7 +1862,7 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
         if (!cl && listed > 0) {
             ProfScope ps(prof, base + "_frontier", 0.0, 0, s);
             fill_async(ccount_.p, nseg, 0, s);
-            hipLaunchKernelGGL(compact_active, dim3(grid_for(listed, 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(compact_active, dim3(grid_for(listed, 256 * kCompactItems)), dim3(256), 0, s,
                                lists_.p, offsets_.p, nseg, listed, act, ccount_.p, clists_.p);
         }
         for (int sr = 0; sr < S; ++sr) {
@@ -1983,9 +1995,9 @@ void LpWorkspace::run(LpMode mode, const DGraph& g, int* label, int* size, int b
                 SubLists L{};
                 for (int b = 0; b < kNumBins; ++b) {
                     const int l = sr * kNumBins + b;
-                    const bool fr = b < kNumBins - 1;  // degree lists: the compacted frontier
-                    L.list[b] = (fr ? clists_.p : lists_.p) + host_offsets[l];
-                    L.dcount[b] = fr ? ccount_.p + l : nullptr;
+                    // every list's compacted frontier (an inactive hub proposes nothing)
+                    L.list[b] = clists_.p + host_offsets[l];
+                    L.dcount[b] = ccount_.p + l;
                     L.hcount[b] = host_counts[l];
                 }
                 const int gs = std::min(gm, 1024);
