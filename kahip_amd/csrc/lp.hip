@@ -32,6 +32,7 @@ namespace {
 constexpr int kWaveCap = 1024;    // LDS hash slots per wave (degree <= 512 => load <= 1/2)
 static_assert(kWaveCap == 1024, "propose_wave loads up to kWaveCap / 2 edges per wave");
 constexpr int kWaveDense = 2048;  // dense LDS counters per wave (refinement, k <= 2048)
+constexpr int kWaveDenseSmall = 512;  // the frontier kernel's counters for k <= 512 (less LDS)
 // Clustering splits the wave list: nodes of degree <= BinConfig::wave_small (64, 128 or 256)
 // come first and use a kernel of 2 x wave_small slots per wave (8 KB of LDS per workgroup at 128
 // instead of 32 KB, so more waves per CU).
@@ -384,14 +385,14 @@ __global__ __launch_bounds__(kBlock) void propose_group(LpArgs a, const int* __r
 // dense counter per label when the labels are the k blocks of the refinement and k is small.
 // One virtual block (kBlock / 64 nodes of the list, a wave each) of the wave kernel; every thread
 // of the workgroup calls it.
-template <LpMode M, bool DENSE, int WCAP>
+template <LpMode M, bool DENSE, int WCAP, int DCAP = kWaveDense>
 __device__ __attribute__((always_inline)) inline void wave_vb(const LpArgs& a,
                                                               const int* __restrict__ list,
                                                               int count, int num_labels, int vb,
                                                               int salt) {
     static_assert(!DENSE || WCAP == kWaveCap, "the dense variant serves every wave degree");
     __shared__ int skey[kBlock / 64][DENSE ? 1 : WCAP];
-    __shared__ int sval[kBlock / 64][DENSE ? kWaveDense : WCAP];
+    __shared__ int sval[kBlock / 64][DENSE ? DCAP : WCAP];  // dense: num_labels <= DCAP
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int gi = vb * (kBlock / 64) + wave;
@@ -527,7 +528,7 @@ struct FrontierLists {
     const int* list[kNumBins];
     const int* dcount[kNumBins];  // dcount[kNumBins - 1] == nullptr: hubs decide elsewhere
 };
-template <int IT, bool DENSE>
+template <int IT, bool DENSE, int DCAP = kWaveDense>
 __global__ __launch_bounds__(kBlock) void propose_frontier(LpArgs a, FrontierLists F,
                                                            int num_labels) {
     constexpr int NPB[kNumBins] = {IT * (kBlock / 8), IT * (kBlock / 16), IT * (kBlock / 32),
@@ -553,7 +554,8 @@ __global__ __launch_bounds__(kBlock) void propose_frontier(LpArgs a, FrontierLis
             case 2: group_vb<LpMode::Refine, 32, IT>(a, F.list[2], cnt[2], vb, t); break;
             case 3: group_vb<LpMode::Refine, 64, IT>(a, F.list[3], cnt[3], vb, t); break;
             case 4:
-                wave_vb<LpMode::Refine, DENSE, kWaveCap>(a, F.list[4], cnt[4], num_labels, vb, t);
+                wave_vb<LpMode::Refine, DENSE, kWaveCap, DCAP>(a, F.list[4], cnt[4], num_labels,
+                                                                vb, t);
                 break;
             default: hubcnt_vb(a, F.list[5], cnt[5], num_labels, vb);
         }
@@ -1461,12 +1463,17 @@ void launch_frontier(int items, bool dense, const LpArgs& a, const FrontierLists
                      int listed, hipStream_t s) {
     // virtual blocks of at least 4 nodes: a listed count bounds the grid
     const dim3 grid(std::min(grid_for(listed, kBlock / 64), kFrontierGrid));
-    switch (items * 2 + (dense ? 1 : 0)) {
-        case 2: hipLaunchKernelGGL((propose_frontier<1, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
-        case 3: hipLaunchKernelGGL((propose_frontier<1, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
-        case 8: hipLaunchKernelGGL((propose_frontier<4, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
-        case 9: hipLaunchKernelGGL((propose_frontier<4, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
-        case 5: hipLaunchKernelGGL((propose_frontier<2, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+    // dense counters of k <= kWaveDenseSmall labels fit a quarter of the LDS
+    const bool small = dense && nl <= kWaveDenseSmall;
+    switch (items * 4 + (dense ? 1 : 0) + (small ? 2 : 0)) {
+        case 4: hipLaunchKernelGGL((propose_frontier<1, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 5: hipLaunchKernelGGL((propose_frontier<1, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 7: hipLaunchKernelGGL((propose_frontier<1, true, kWaveDenseSmall>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 16: hipLaunchKernelGGL((propose_frontier<4, false>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 17: hipLaunchKernelGGL((propose_frontier<4, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 19: hipLaunchKernelGGL((propose_frontier<4, true, kWaveDenseSmall>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 9: hipLaunchKernelGGL((propose_frontier<2, true>), grid, dim3(kBlock), 0, s, a, F, nl); break;
+        case 11: hipLaunchKernelGGL((propose_frontier<2, true, kWaveDenseSmall>), grid, dim3(kBlock), 0, s, a, F, nl); break;
         default: hipLaunchKernelGGL((propose_frontier<2, false>), grid, dim3(kBlock), 0, s, a, F, nl);
     }
 }
