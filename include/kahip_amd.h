@@ -28,7 +28,8 @@ typedef struct {
     int ip_repetitions;  /* initial-partitioning repetitions; -1 = the reference's count */
     int ip_threads;      /* CPU threads of the initial partitioner; 0 = automatic */
     int bins[6];         /* upper degree bounds of the LP kernel bins; bins[4] <= 512 */
-    int profile;         /* record per-launch timings (kahip_profile_json) */
+    int profile;         /* record timings (kahip_profile_json): 1 every launch, 2 only the
+                            phases and the dominant clustering kernel */
     int rating;          /* edge rating of the LP clustering: KAHIP_RATING_* */
     int rating_levels;   /* the rating applies to coarsening levels < rating_levels */
     const float* mlp_weights; /* KAHIP_MLP_PARAMS floats (host), for KAHIP_RATING_LEARNED */

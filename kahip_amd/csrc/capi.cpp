@@ -134,7 +134,8 @@ PartitionParams to_params(const kahip_params* q) {
                "wave_small must be 64, 128 or 256");
     p.bins.group_items = q->group_items;
     p.bins.wave_small = q->wave_small;
-    p.profile = q->profile != 0;
+    KH_REQUIRE(q->profile >= 0 && q->profile <= 2, "profile must be 0, 1 or 2");
+    p.profile = q->profile;
     KH_REQUIRE(q->rating >= 0 && q->rating <= 2, "bad rating");
     p.rating = (RatingMode)q->rating;
     p.rating_levels = q->rating_levels;

@@ -54,6 +54,13 @@ struct DevBuf {
 // achieved GB/s = bytes / duration is measured on the stream the kernel ran on.
 struct Profiler {
     bool enabled = false;
+    // phases only: the whole clustering of a level, the contraction, renumbering and cut, and
+    // the dominant clustering kernel (few event pairs, so the timing barely perturbs the run)
+    bool light = false;
+    static bool phase_scope(const std::string& n) {
+        return n == "lp_cluster_total" || n == "lp_cluster_propose_hub" || n == "contract" ||
+               n == "remap" || n == "reorder" || n == "edge_cut";
+    }
     struct Rec {
         std::string name;
         hipEvent_t a, b;
@@ -76,7 +83,7 @@ struct Profiler {
         return pool[pool_used++];
     }
     int begin(const std::string& name, double bytes, long long edges, hipStream_t s) {
-        if (!enabled) return -1;
+        if (!enabled || (light && !phase_scope(name))) return -1;
         Rec r{name, take(), take(), bytes, edges};
         KH_CHECK(hipEventRecord(r.a, s));
         recs.push_back(r);

@@ -160,7 +160,8 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
     S = PartitionStats();
     hipStream_t s = s_;
     Profiler* prof = p.profile ? &this->prof : nullptr;
-    this->prof.enabled = p.profile;
+    this->prof.enabled = p.profile != 0;
+    this->prof.light = p.profile == 2;
     this->prof.reset();
 
     const unsigned U = upper_bound_for(input.total_vwgt, p.k, p.imbalance);
@@ -235,6 +236,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             S.ms_rating += ev.ms();
             rated.adjwgt = score.p;
         }
+        ProfScope ps_cluster(prof, "lp_cluster_total", 0.0, 0, s);  // the level's clustering
         lp.prepare(LpMode::Cluster, rated, fine.n, p.subrounds, cseed, p.bins, s);
         lp.run(LpMode::Cluster, rated, label.p, size.p, (int)B, p.cluster_rounds, cseed, s, prof,
                "lp_cluster");
@@ -255,6 +257,7 @@ void Engine::partition(const DGraph& input, const PartitionParams& p, int* part_
             S.edges_cluster += edges_round * p.cluster_rounds;
             overlay(fine.n, label.p, elabel.p, s);
         }
+        ps_cluster.close();
         if ((int)maps.size() <= level) maps.emplace_back();
         if ((int)coarse.size() <= level) coarse.emplace_back(new OwnedGraph());
         maps[level].reserve(fine.n);

@@ -356,6 +356,24 @@ def test_cached_workspaces_are_reused_across_graphs(G):
     check()
 
 
+def test_profile_modes(G):
+    """profile=1 times every launch; profile=2 only the phases and the dominant clustering kernel
+    (same partition either way); other values are rejected."""
+    x, a, vw, ew = G["rmat12"]
+    want, _ = K.partition(x, a, vw, ew, k=16, seed=5)
+    full, _ = K.partition(x, a, vw, ew, k=16, seed=5, profile=1)
+    full_names = set(K.profile())
+    light, _ = K.partition(x, a, vw, ew, k=16, seed=5, profile=2)
+    light_names = set(K.profile())
+    assert np.array_equal(full, want) and np.array_equal(light, want)
+    allowed = {"lp_cluster_total", "lp_cluster_propose_hub", "contract", "remap", "reorder",
+               "edge_cut"}
+    assert {"lp_cluster_total", "contract", "edge_cut"} <= light_names <= allowed
+    assert light_names < full_names
+    with pytest.raises(K.KahipError, match="profile"):
+        K.partition(x, a, vw, ew, k=16, seed=5, profile=3)
+
+
 GRAPH_OFF = ("hipGraph-captured LP rounds are disabled: that path hit a host segmentation fault "
              "under rocprofv3 whose cause was not found")
 
