@@ -345,9 +345,9 @@ def main():
     prof = {}
     t_start = time.perf_counter()
     for i in range(args.steps):
-        # every timed step records hipEvents around each phase and each launch of the dominant
-        # kernel (the roofline's kernel durations); the per-launch events of every other kernel
-        # (profile=1) would cost about 6 ms per step, so they come from an untimed step below
+        # every timed step records hipEvents around each phase and each clustering proposal
+        # launch (the roofline's kernel durations); events around every other launch too
+        # (profile=1) would cost a few ms per step
         _, st = g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=2)
         stats.append(st)
         for name, v in K.profile().items():
@@ -386,9 +386,7 @@ def main():
                     "traffic": measured_traffic(args.workload, dom_name),
                     "kernel": dom_name, "launches": d["launches"],
                     "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(per_launch_ms * 1e3, 2)}
-    # every clustering proposal kernel, from one untimed step with per-launch events
-    g.partition(k=w["k"], imbalance=0.03, seed=0, want_part=False, profile=1)
-    lp_all = [v for k, v in K.profile().items() if k.startswith("lp_cluster_propose")]
+    lp_all = [v for k, v in prof.items() if k.startswith("lp_cluster_propose")]
     lp_gbs = (sum(v["bytes"] for v in lp_all) / (sum(v["ms"] for v in lp_all) / 1e3) / 1e9
               if lp_all else None)
 

@@ -29,7 +29,7 @@ typedef struct {
     int ip_threads;      /* CPU threads of the initial partitioner; 0 = automatic */
     int bins[6];         /* upper degree bounds of the LP kernel bins; bins[4] <= 512 */
     int profile;         /* record timings (kahip_profile_json): 1 every launch, 2 only the
-                            phases and the dominant clustering kernel */
+                            phases and the clustering proposal kernels */
     int rating;          /* edge rating of the LP clustering: KAHIP_RATING_* */
     int rating_levels;   /* the rating applies to coarsening levels < rating_levels */
     const float* mlp_weights; /* KAHIP_MLP_PARAMS floats (host), for KAHIP_RATING_LEARNED */

@@ -54,12 +54,15 @@ struct DevBuf {
 // achieved GB/s = bytes / duration is measured on the stream the kernel ran on.
 struct Profiler {
     bool enabled = false;
-    // phases only: the whole clustering of a level, the contraction, renumbering and cut, and
-    // the dominant clustering kernel (few event pairs, so the timing barely perturbs the run)
+    // phases and proposal kernels only: the whole clustering of a level, the contraction,
+    // renumbering and cut, and every clustering proposal launch (the candidates for the dominant
+    // kernel); fewer event pairs, so the timing perturbs the run less
     bool light = false;
     static bool phase_scope(const std::string& n) {
-        return n == "lp_cluster_total" || n == "lp_cluster_propose_hub" || n == "contract" ||
-               n == "remap" || n == "reorder" || n == "edge_cut";
+        static const std::string prop = "lp_cluster_propose_";
+        if (n.compare(0, prop.size(), prop) == 0) return n != "lp_cluster_propose_hub_merge";
+        return n == "lp_cluster_total" || n == "contract" || n == "remap" || n == "reorder" ||
+               n == "edge_cut";
     }
     struct Rec {
         std::string name;

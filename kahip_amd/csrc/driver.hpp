@@ -34,7 +34,7 @@ struct PartitionParams {
     int fm_rounds = 0;         // > 0: CPU pairwise FM on every level after its LP refinement
     int graph_refine = 0;      // must be 0: hipGraph-captured refinement rounds are disabled
     BinConfig bins;
-    int profile = 0;  // 0 off, 1 every launch, 2 phases and the dominant clustering kernel
+    int profile = 0;  // 0 off, 1 every launch, 2 phases and the clustering proposal kernels
     // Edge rating that replaces the edge weights in the LP clustering of levels < rating_levels
     // (None: the edge weights themselves, as in the FASTSOCIAL preset).
     RatingMode rating = RatingMode::None;

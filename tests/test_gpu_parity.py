@@ -357,8 +357,8 @@ def test_cached_workspaces_are_reused_across_graphs(G):
 
 
 def test_profile_modes(G):
-    """profile=1 times every launch; profile=2 only the phases and the dominant clustering kernel
-    (same partition either way); other values are rejected."""
+    """profile=1 times every launch; profile=2 only the phases and the clustering proposal
+    kernels (same partition either way); other values are rejected."""
     x, a, vw, ew = G["rmat12"]
     want, _ = K.partition(x, a, vw, ew, k=16, seed=5)
     full, _ = K.partition(x, a, vw, ew, k=16, seed=5, profile=1)
@@ -366,9 +366,12 @@ def test_profile_modes(G):
     light, _ = K.partition(x, a, vw, ew, k=16, seed=5, profile=2)
     light_names = set(K.profile())
     assert np.array_equal(full, want) and np.array_equal(light, want)
-    allowed = {"lp_cluster_total", "lp_cluster_propose_hub", "contract", "remap", "reorder",
-               "edge_cut"}
-    assert {"lp_cluster_total", "contract", "edge_cut"} <= light_names <= allowed
+    allowed = {"lp_cluster_total", "contract", "remap", "reorder", "edge_cut"}
+    proposals = {n for n in light_names if n.startswith("lp_cluster_propose_")}
+    assert {"lp_cluster_total", "contract", "edge_cut"} <= light_names
+    assert light_names - proposals <= allowed and "lp_cluster_propose_hub_merge" not in proposals
+    assert proposals == {n for n in full_names if n.startswith("lp_cluster_propose_")} - {
+        "lp_cluster_propose_hub_merge"}
     assert light_names < full_names
     with pytest.raises(K.KahipError, match="profile"):
         K.partition(x, a, vw, ew, k=16, seed=5, profile=3)
