@@ -356,6 +356,31 @@ def test_cached_workspaces_are_reused_across_graphs(G):
     check()
 
 
+def test_concurrent_partitions_use_separate_engines(G):
+    """Two threads partitioning at once: one holds the device's cached engine, the other runs on
+    an engine of its own (the call releases the GIL); both give the mirror's partition."""
+    import threading
+    x1, a1, vw1, ew1 = G["rmat12"]
+    x2, a2, vw2, ew2 = G["rgg15"]
+    want1, _ = mirror_partition(x1, a1, vw1, ew1, 16, 0.03, 5)
+    want2, _ = mirror_partition(x2, a2, vw2, ew2, 8, 0.03, 1)
+    got = {}
+
+    def work(key, args, k, seed, reps=4):
+        for i in range(reps):
+            got[(key, i)] = K.partition(*args, k=k, imbalance=0.03, seed=seed)[0]
+
+    t1 = threading.Thread(target=work, args=(1, (x1, a1, vw1, ew1), 16, 5))
+    t2 = threading.Thread(target=work, args=(2, (x2, a2, vw2, ew2), 8, 1))
+    t1.start()
+    t2.start()
+    t1.join()
+    t2.join()
+    for i in range(4):
+        assert np.array_equal(got[(1, i)], want1)
+        assert np.array_equal(got[(2, i)], want2)
+
+
 def test_profile_modes(G):
     """profile=1 times every launch; profile=2 only the phases and the clustering proposal
     kernels (same partition either way); other values are rejected."""
