@@ -84,6 +84,19 @@ __device__ inline bool better(int ac, uint32_t at, int al, int bc, uint32_t bt, 
     return ac > bc || (ac == bc && (at > bt || (at == bt && al < bl)));
 }
 
+// Candidate (conn, lab) of node v (weight nw) against the best so far: the label's size (and
+// bound) is read only when the candidate would beat c, so a thread reads the sizes of its running
+// maxima only. The result, the maximum of better() over the eligible labels, is the same as with
+// every size read: c only ever holds eligible labels, and the best eligible one beats whatever c
+// holds when it comes up.
+__device__ inline void consider(const LpArgs& a, int v, int nw, int conn, int lab, Cand& c) {
+    const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
+    if (!better(conn, t, lab, c.conn, c.tie, c.lab)) return;
+    const int sz = a.size[lab];
+    if ((long long)sz + nw >= (long long)bound_of(a, lab)) return;
+    c = Cand{conn, t, lab, sz};
+}
+
 template <int W>
 __device__ inline void reduce_cand(Cand& c) {
 #pragma unroll
@@ -822,10 +835,7 @@ __global__ __launch_bounds__(THREADS) void propose_hub_chunk(
             conn_own = conn;
             continue;
         }
-        const int sz = a.size[lab];
-        if ((long long)sz + nw >= (long long)bound_of(a, lab)) continue;
-        const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-        if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+        consider(a, v, nw, conn, lab, c);
     }
     block_reduce<THREADS>(conn_own, c);
     if (threadIdx.x == 0) {
@@ -949,10 +959,7 @@ void hub_bucket_merge(
                 conn_own = conn;
                 continue;
             }
-            const int sz = a.size[lab];
-            if ((long long)sz + nw >= (long long)bound_of(a, lab)) continue;
-            const uint32_t t = hash3(a.tie_seed, (uint32_t)v, (uint32_t)lab);
-            if (better(conn, t, lab, c.conn, c.tie, c.lab)) c = Cand{conn, t, lab, sz};
+            consider(a, v, nw, conn, lab, c);
         }
         block_reduce<THREADS>(conn_own, c);
         if (threadIdx.x == 0) {
